@@ -1,0 +1,236 @@
+/*
+ * mivegas.h - C ABI of the MI355X (gfx950) VEGAS engine, libmivegas.so.
+ *
+ * The library replaces the TensorFlow graph bodies of vegasflow's hot path
+ * (vegasflow 1.4.0, src/vegasflow/...). Each entry point names the reference
+ * function it stands in for. All pointers are device pointers owned by the
+ * caller (the PyTorch caching allocator in the Python host code); the library
+ * never allocates or frees device memory and never synchronises: every call
+ * enqueues work on the given HIP stream and returns.
+ *
+ * Return value: >= 0 on success (some calls return a count), or a negative
+ * mv_status code; mv_last_error() then holds a message (thread-local).
+ *
+ * Conventions
+ *   - grids are (n_dim, MV_BINS + 1) FP64 row-major, edges in [0, 1];
+ *   - bin indices are uint8 (n_events, n_dim) row-major;
+ *   - partial sums travel in rows of mv_partial_stride(...) doubles
+ *     [sum_k, sum_k^2, histogram]. A producer (fused / digest call) has
+ *     every workgroup write one block row; the last workgroup to finish in
+ *     each group of MV_GROUP consecutive workgroups (a ticket counter, reset
+ *     by that workgroup) adds the group's block rows in a fixed order into
+ *     one group row. Producers return the number of group rows, which the
+ *     finalize calls reduce, again in a fixed order.
+ */
+#ifndef MIVEGAS_H
+#define MIVEGAS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MV_ABI_VERSION 1
+#define MV_BINS 50             /* BINS_MAX, configflow.py:13 */
+#define MV_FUSED_MAX_DIM 16    /* largest n_dim the fused kernels take */
+#define MV_MAX_COMP 4          /* components of the built-in mixture family */
+#define MV_MAX_OUT 16          /* outputs of a vector integrand (digest) */
+#define MV_MAX_BLOCKS 2048     /* workgroups of one producer launch */
+#define MV_GROUP 64            /* workgroups folded into one group row */
+#define MV_MAX_GROUPS (MV_MAX_BLOCKS / MV_GROUP)
+#define MV_DIGEST_DIMS 16      /* dimensions per grid.y slice of the digest */
+#define MV_MAX_SLICES 64       /* digest slices: n_dim <= 1024 when training */
+#define MV_TICKETS (MV_MAX_GROUPS * MV_MAX_SLICES)
+
+typedef enum {
+  MV_OK = 0,
+  MV_ERR_ARG = -1,
+  MV_ERR_UNSUPPORTED = -2,
+  MV_ERR_LAUNCH = -3,
+  MV_ERR_WORKSPACE = -4,
+  MV_ERR_DEVICE = -5
+} mv_status;
+
+typedef enum { MV_F64 = 0, MV_MIXED = 1 } mv_precision;
+
+typedef enum {
+  MV_FN_CONSTANT = 0, /* f = c                                               */
+  MV_FN_GAUSSIAN = 1, /* (1/(a sqrt(pi)))^d exp(-sum((x_j - c_j)/a)^2)       */
+  MV_FN_PRODUCT = 2,  /* prod_j x_j                                          */
+  MV_FN_GAUSSMIX = 3, /* sum_k w_k gaussian(x; a_k, c_k)                     */
+  MV_FN_BWPOWER = 4   /* (G/2pi)/((x_0-M)^2 + G^2/4) * x_1^p, rest flat      */
+} mv_family;
+
+/* Built-in integrand: evaluated in registers inside the fused kernels.
+ * comp[k] = {weight, 1/a, prefactor, unused} for the Gaussian families,
+ * comp[0] = {mass, width/2, (width/2)/pi, power} for MV_FN_BWPOWER,
+ * comp[0][0] = c for MV_FN_CONSTANT. centre[k][j] is component k's centre
+ * in dimension j. */
+typedef struct {
+  int32_t family;
+  int32_t n_comp;
+  int32_t reserved[2];
+  double comp[MV_MAX_COMP][4];
+  double centre[MV_MAX_COMP][MV_FUSED_MAX_DIM];
+} mv_integrand;
+
+/* One launch worth of events of an iteration.
+ * Uniform draws: Philox4x32-10, key = seed, counter = (event_lo, event_hi,
+ * draw_id, block) with event = event_offset + local index. In MV_F64 a draw
+ * is r = ((hi:lo) >> 12) * 2^-52 mapped to [1e-8, 1 - 1e-8]
+ * (monte_carlo.py:264-266); in MV_MIXED it is the 32-bit word hi alone.
+ * ``inject`` (nullable) replaces the generator: FP64 uniforms (n_events,
+ * n_dim) in MV_F64, uint32 hi words (n_events, n_dim) in MV_MIXED. */
+typedef struct {
+  int32_t n_dim;
+  int32_t precision;     /* mv_precision */
+  int32_t train;         /* accumulate the bin histogram (vflow.py:376) */
+  int32_t accumulate;    /* add into the group rows instead of overwriting
+                            them (second and later steps of an iteration) */
+  int64_t n_events;      /* events in this launch */
+  int64_t event_offset;  /* global index of its first event */
+  uint64_t seed;
+  uint32_t draw_id;
+  uint32_t reserved2;
+  double xjac;           /* weight factor: 1/N (vegas, plain), 1/n_cubes (plus) */
+  double xdeltajac;      /* prod(xmax - xmin), 1 without limits */
+  const double* xmin;    /* nullable: integration limits (n_dim) */
+  const double* xdelta;  /* nullable: xmax - xmin (n_dim) */
+  const void* inject;    /* nullable, see above */
+} mv_step;
+
+/* Caller-owned partial-sum workspace of a producer launch. tickets must be
+ * zero when first used (e.g. allocated zeroed); every launch leaves them
+ * zero again. Calls that share one workspace must be stream-ordered. */
+typedef struct {
+  double* block_rows;   /* MV_MAX_BLOCKS rows, scratch */
+  double* group_rows;   /* MV_MAX_GROUPS rows, the producer's output */
+  int32_t* tickets;     /* MV_TICKETS counters */
+  int64_t stride;       /* doubles per row, >= mv_partial_stride(...) */
+} mv_partials;
+
+/* Stratification of VegasFlowPlus (vflowplus.py:113-139). Hypercube c has
+ * coordinates given by the base-n_strat digits of c, last dimension fastest
+ * (itertools.product order); offsets is the exclusive prefix sum of n_ev
+ * (n_cubes + 1 entries, offsets[n_cubes] = events of the iteration). */
+typedef struct {
+  int32_t n_strat;
+  int32_t n_cubes;
+  const int32_t* n_ev;
+  const int32_t* offsets;
+} mv_strata;
+
+/* Reduction of partial rows into iteration results (vflow.py:432-442,
+ * plain.py:37-43): res_k = sum tmp_k, sigma_k from N and sum tmp_k^2. */
+typedef struct {
+  int32_t n_dim;
+  int32_t n_out;         /* outputs per event (1 for a scalar integrand) */
+  int32_t train;         /* rows carry a histogram; refine the grid */
+  int32_t refine;        /* apply the refinement to ``grid`` */
+  int32_t raw;           /* write sum tmp^2 instead of sigma */
+  int32_t reserved;
+  double n_events;       /* N of the sigma formula */
+} mv_finalize;
+
+/* ---- library ---------------------------------------------------------- */
+int mv_abi_version(void);
+const char* mv_last_error(void);
+/* sizeof of the ABI structs, for binding-layout checks:
+ * 0 mv_integrand, 1 mv_step, 2 mv_partials, 3 mv_strata, 4 mv_finalize. */
+int64_t mv_struct_size(int32_t which);
+/* Doubles per partial row: 2 * n_out + (train ? n_dim * MV_BINS : 0). */
+int64_t mv_partial_stride(int32_t n_dim, int32_t n_out, int32_t train);
+/* Workgroups a producer launches for ``n_events`` events, and the group
+ * rows it then writes. */
+int32_t mv_producer_blocks(int64_t n_events);
+int32_t mv_producer_groups(int64_t n_events);
+
+/* ---- VegasFlow ---------------------------------------------------------- */
+/* Fused iteration body for built-in integrands: draws, grid map and
+ * Jacobian (vflow.py:87-126 + 33-83, monte_carlo.py:249-275), integrand,
+ * w*f sums (vflow.py:416-421) and the per-dimension (w*f)^2 bin histogram
+ * (vflow.py:370-387, utils.py:10-44), without any per-event HBM traffic.
+ * Returns the number of group rows. Requires n_dim <= MV_FUSED_MAX_DIM. */
+int mv_vegas_fused(const mv_step* step, const mv_integrand* fn, const double* grid,
+                   const mv_partials* parts, void* stream);
+
+/* Sampling half of the unfused path (vflow.py:87-126, monte_carlo.py:249-275):
+ * x (n, d) and w (n,) in the step precision (double or float), bin indices
+ * idx (n, d) uint8 (nullable when not training). Any n_dim. */
+int mv_vegas_sample(const mv_step* step, const double* grid, void* x, void* w,
+                    uint8_t* idx, void* stream);
+
+/* Digest half of the unfused path (vflow.py:414-430): tmp = w * f for a
+ * (n, n_out) integrand f with row stride f_stride (elements), histogram of
+ * tmp[:, main_dim]^2 by idx. f and w are double (MV_F64) or float
+ * (MV_MIXED); w_stride 0 broadcasts w[0] (plain.py:18-35). Any n_dim.
+ * Returns the number of group rows. */
+int mv_vegas_digest(const mv_step* step, const void* f, int64_t f_stride, int32_t n_out,
+                    int32_t main_dim, const void* w, int64_t w_stride, const uint8_t* idx,
+                    const mv_partials* parts, void* stream);
+
+/* Reduce ``n_rows`` group rows (fixed order) into out[0:n_out] = res and
+ * out[n_out:2 n_out] = sigma (vflow.py:437-438, plain.py:37-43),
+ * hist_out (nullable, (n_dim, MV_BINS)) = arr_res2, and when fz->refine
+ * refine ``grid`` in place (vflow.py:349-362 + 129-211). */
+int mv_vegas_finalize(const mv_finalize* fz, const double* rows, int64_t row_stride,
+                      int32_t n_rows, double* grid, double* out, double* hist_out,
+                      void* stream);
+
+/* refine_grid_per_dimension (vflow.py:129-211) for every dimension:
+ * hist (n_dim, MV_BINS) with row stride hist_stride, grid (n_dim,
+ * MV_BINS + 1) refined in place. */
+int mv_refine_grid(int32_t n_dim, const double* hist, int64_t hist_stride, double* grid,
+                   void* stream);
+
+/* consume_array_into_indices (utils.py:10-44): out[b] = sum of input[i]
+ * with indices[i] == b, b < result_size; out is overwritten. */
+int mv_consume_array_into_indices(const double* input, const int32_t* indices, int64_t n,
+                                  int32_t result_size, double* out, void* stream);
+
+/* ---- PlainFlow ---------------------------------------------------------- */
+/* Fused plain MC (plain.py:18-35) for built-in integrands; rows without a
+ * histogram. */
+int mv_plain_fused(const mv_step* step, const mv_integrand* fn, const mv_partials* parts,
+                   void* stream);
+/* Uniform points (monte_carlo.py:264-274) for the unfused plain path:
+ * x (n, d) in the step precision. */
+int mv_plain_sample(const mv_step* step, void* x, void* stream);
+
+/* ---- VegasFlowPlus ------------------------------------------------------ */
+/* Fused VEGAS+ body (vflowplus.py:37-80 + 187-220) for built-in integrands.
+ * Adds per-cube sums into ress / ress2 (n_cubes; zero before the first
+ * iteration, zeroed again by mv_vplus_finalize) and writes histogram rows
+ * (n_out = 1). step->n_events is an upper bound of the events; the kernel
+ * takes the actual count from strata->offsets[n_cubes]. */
+int mv_vplus_fused(const mv_step* step, const mv_strata* strata, const mv_integrand* fn,
+                   const double* grid, double* ress, double* ress2, const mv_partials* parts,
+                   void* stream);
+/* Sampling half (vflowplus.py:37-80): x, w, idx as mv_vegas_sample plus the
+ * cube of each event seg (n,) int32. */
+int mv_vplus_sample(const mv_step* step, const mv_strata* strata, const double* grid,
+                    void* x, void* w, uint8_t* idx, int32_t* seg, void* stream);
+/* Digest half (vflowplus.py:209-218): per-cube sums of w*f and (w*f)^2 into
+ * ress / ress2, histogram rows. */
+int mv_vplus_digest(const mv_step* step, const mv_strata* strata, const void* f,
+                    const void* w, const uint8_t* idx, const int32_t* seg, double* ress,
+                    double* ress2, const mv_partials* parts, void* stream);
+/* Iteration end (vflowplus.py:222-242, 153-163): out = {res, sigma,
+ * events of the next iteration}; when adaptive, n_ev and offsets are
+ * rewritten for the next iteration (damped = max(arr_var, 0)^(beta/2),
+ * n_ev = int(max(min_neval, damped * init_calls / 2 / sum damped))); ress
+ * and ress2 are zeroed; with train the histogram rows are reduced into
+ * hist_out (nullable) and, with refine, the grid is refined. arr_var_out
+ * (nullable, n_cubes) receives ress2 * n_ev - ress^2. */
+int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t* n_ev, int32_t* offsets,
+                      double* ress, double* ress2, int32_t adaptive, int32_t min_neval,
+                      double init_calls, double beta, const double* rows, int64_t row_stride,
+                      int32_t n_rows, int32_t train, int32_t refine, double* grid, double* out,
+                      double* arr_var_out, double* hist_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIVEGAS_H */

@@ -1,0 +1,797 @@
+// vegas.hip - VegasFlow and PlainFlow kernels for gfx950 and their C ABI.
+//
+// Kernels
+//   vegas_fused_f64 / vegas_fused_mixed   draws + grid map + built-in
+//       integrand + sums + per-wave LDS bin histograms; no per-event HBM
+//       traffic. Each workgroup writes one block row of partial sums.
+//   vegas_sample_{f64,mixed} / vegas_digest   the unfused path around a
+//       torch-callable integrand (x, w, idx through HBM).
+//   finalize_kernel   fixed-order reduction of group rows, sigma, and the
+//       per-dimension refinement (one workgroup per dimension).
+//   plain_*   PlainFlow variants.
+#include <algorithm>
+#include <type_traits>
+
+#include "abi_util.hpp"
+#include "device.hpp"
+#include "refine.hpp"
+#include "rows.hpp"
+
+namespace mv {
+
+std::string& last_error_slot() {
+  static thread_local std::string s;
+  return s;
+}
+
+__device__ __forceinline__ int clamp_bin(int b) { return b < 0 ? 0 : (b > kBins - 1 ? kBins - 1 : b); }
+
+// Block row [sum, sum2, hist(d*50)] (n_out = 1) from per-wave registers and
+// per-wave LDS histograms, then the group fold.
+template <bool TRAIN>
+__device__ __forceinline__ void finish_block(double acc, double acc2, const double* hist,
+                                             int hist_len, const RowsK& R) {
+  __shared__ double red[2 * kWaves];
+  acc = wave_sum(acc);
+  acc2 = wave_sum(acc2);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wave] = acc;
+    red[kWaves + wave] = acc2;
+  }
+  __syncthreads();
+  double* row = R.blk + (int64_t)blockIdx.x * R.stride;
+  if (threadIdx.x == 0) {
+    double a = red[0], b = red[kWaves];
+    for (int k = 1; k < kWaves; ++k) {
+      a += red[k];
+      b += red[kWaves + k];
+    }
+    row[0] = a;
+    row[1] = b;
+  }
+  if (TRAIN) {
+    for (int i = threadIdx.x; i < hist_len; i += kBlock) {
+      double h = hist[i];
+      for (int k = 1; k < kWaves; ++k) h += hist[k * hist_len + i];
+      row[2 + i] = h;
+    }
+  }
+  group_fold(R, 2 + (TRAIN ? hist_len : 0));
+}
+
+// ---------------------------------------------------------------------------
+// fused VegasFlow, FP64 (vflow.py:87-126 + 33-83 + 389-430,
+// monte_carlo.py:249-275)
+// LDS: per (dim, bin) the map frac -> y = P + Q frac (Integrand::fold), the
+// Jacobian factor W = dx * 50, and kWaves private histograms.
+// ---------------------------------------------------------------------------
+template <int FAM, bool TRAIN>
+__global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand fn,
+                                                          const double* __restrict__ grid,
+                                                          RowsK R) {
+  extern __shared__ double smem[];
+  const int d = s.n_dim;
+  const int nb = d * kBins;
+  double2* tab = reinterpret_cast<double2*>(smem);
+  double* tw = smem + 2 * nb;
+  double* hist = tw + nb;
+  const bool limits = s.xmin != nullptr;
+  for (int i = threadIdx.x; i < nb; i += kBlock) {
+    const int j = i / kBins, b = i - j * kBins;
+    const double lo = grid[j * (kBins + 1) + b];
+    const double dx = grid[j * (kBins + 1) + b + 1] - lo;  // vflow.py:73
+    double P = lo, Q = dx;
+    if (limits) {  // monte_carlo.py:273
+      P = s.xmin[j] + lo * s.xdelta[j];
+      Q = dx * s.xdelta[j];
+    }
+    Integrand<double, FAM>::fold(fn, j, P, Q);
+    tab[i] = make_double2(P, Q);
+    tw[i] = dx * kFBins;  // vflow.py:78
+  }
+  if (TRAIN)
+    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+  __syncthreads();
+  double* wh = hist + (threadIdx.x >> 6) * nb;
+
+  Integrand<double, FAM> F(fn);
+  const double* inj = static_cast<const double*>(s.inject);
+  const double wscale = limits ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
+  double acc = 0.0, acc2 = 0.0;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
+    const uint64_t ge = (uint64_t)(s.event_offset + i);
+    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    F.begin();
+    double w = 1.0;
+    uint32_t pk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (g < s.groups) {
+        uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
+        if (!inj) {
+          h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+          l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = 4 * g + q;
+          if (j < d) {
+            const double u = inj ? inj[i * d + j] : draw_f64(word(h, q), word(l, q));
+            const double xn = kFBins * (1.0 - u);           // vflow.py:117
+            const int ind = clamp_bin((int)xn);             // vflow.py:67
+            const double2 t = tab[j * kBins + ind];
+            const double frac = xn - floor(xn);             // vflow.py:75
+            F.add_folded(j, fma(t.y, frac, t.x));           // vflow.py:76
+            w = w * tw[j * kBins + ind];                    // vflow.py:78
+            if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
+          }
+        }
+      }
+    }
+    const double tmp = (w * wscale) * F.value();  // vflow.py:416
+    const double tmp2 = tmp * tmp;                // vflow.py:417
+    acc += tmp;
+    acc2 += tmp2;
+    if (TRAIN) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (g < s.groups) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j = 4 * g + q;
+            if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
+          }
+        }
+      }
+    }
+  }
+  finish_block<TRAIN>(acc, acc2, hist, nb, R);
+}
+
+// ---------------------------------------------------------------------------
+// fused VegasFlow, mixed precision: FP32 map and integrand from the 32 high
+// bits of each draw, FP64 sums and histograms.
+// Bin and fraction come from the 32x32->64 product s * 50 with s = ~hi,
+// i.e. xn = 50 * (1 - u) in fixed point, which keeps ind in [0, 49].
+// ---------------------------------------------------------------------------
+template <int FAM, bool TRAIN>
+__global__ __launch_bounds__(kBlock) void vegas_fused_mixed(StepK s, mv_integrand fn,
+                                                            const double* __restrict__ grid,
+                                                            RowsK R) {
+  extern __shared__ double smem[];
+  const int d = s.n_dim;
+  const int nb = d * kBins;
+  float2* tab = reinterpret_cast<float2*>(smem);          // nb float2
+  float* tw = reinterpret_cast<float*>(smem + nb);         // nb float (nb/2 doubles)
+  double* hist = smem + nb + (nb + 1) / 2;
+  const bool limits = s.xmin != nullptr;
+  for (int i = threadIdx.x; i < nb; i += kBlock) {
+    const int j = i / kBins, b = i - j * kBins;
+    const double lo = grid[j * (kBins + 1) + b];
+    const double dx = grid[j * (kBins + 1) + b + 1] - lo;
+    double P = lo, Q = dx;
+    if (limits) {
+      P = s.xmin[j] + lo * s.xdelta[j];
+      Q = dx * s.xdelta[j];
+    }
+    Integrand<float, FAM>::fold(fn, j, P, Q);
+    tab[i] = make_float2((float)P, (float)Q);
+    tw[i] = (float)(dx * kFBins);
+  }
+  if (TRAIN)
+    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+  __syncthreads();
+  double* wh = hist + (threadIdx.x >> 6) * nb;
+
+  Integrand<float, FAM> F(fn);
+  const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
+  const double wscale = limits ? s.xjac * s.xdeltajac : s.xjac;
+  double acc = 0.0, acc2 = 0.0;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
+    const uint64_t ge = (uint64_t)(s.event_offset + i);
+    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    F.begin();
+    float w = 1.0f;
+    uint32_t pk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (g < s.groups) {
+        uint4 h = make_uint4(0, 0, 0, 0);
+        if (!inj) h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = 4 * g + q;
+          if (j < d) {
+            const uint32_t hw = inj ? inj[i * d + j] : word(h, q);
+            const uint64_t p = (uint64_t)(~hw) * (uint64_t)kBins;
+            const int ind = (int)(p >> 32);
+            const float frac = (float)(((uint32_t)p) >> 8) * 0x1p-24f;
+            const float2 t = tab[j * kBins + ind];
+            F.add_folded(j, fmaf(t.y, frac, t.x));
+            w = w * tw[j * kBins + ind];
+            if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
+          }
+        }
+      }
+    }
+    // w * f in FP32; the 1/N (and limits) factor in FP64 so that its
+    // rounding is not a systematic bias
+    const double tmp = (double)(w * F.value()) * wscale;
+    const double tmp2 = tmp * tmp;
+    acc += tmp;
+    acc2 += tmp2;
+    if (TRAIN) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (g < s.groups) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j = 4 * g + q;
+            if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
+          }
+        }
+      }
+    }
+  }
+  finish_block<TRAIN>(acc, acc2, hist, nb, R);
+}
+
+// ---------------------------------------------------------------------------
+// fused PlainFlow (plain.py:18-35)
+// ---------------------------------------------------------------------------
+template <int FAM, bool MIXED>
+__global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, RowsK R) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  Integrand<T, FAM> F(fn);
+  const bool limits = s.xmin != nullptr;
+  const double wt = limits ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
+  double acc = 0.0, acc2 = 0.0;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
+    const uint64_t ge = (uint64_t)(s.event_offset + i);
+    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    F.begin();
+    for (int g = 0; g < s.groups; ++g) {
+      uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
+      if (!s.inject) {
+        h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+        if (!MIXED) l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = 4 * g + q;
+        if (j < s.n_dim) {
+          T x;
+          if (MIXED) {
+            const uint32_t hw = s.inject ? static_cast<const uint32_t*>(s.inject)[i * s.n_dim + j]
+                                         : word(h, q);
+            float xf = fmaf((float)(hw >> 9), 0x1p-23f, 0x1p-24f);
+            if (limits) xf = fmaf(xf, (float)s.xdelta[j], (float)s.xmin[j]);
+            x = (T)xf;
+          } else {
+            const double u = s.inject ? static_cast<const double*>(s.inject)[i * s.n_dim + j]
+                                      : draw_f64(word(h, q), word(l, q));
+            x = (T)(limits ? s.xmin[j] + u * s.xdelta[j] : u);
+          }
+          F.add(j, x);
+        }
+      }
+    }
+    const double tmp = (double)F.value() * wt;  // plain.py:27
+    const double tmp2 = tmp * tmp;
+    acc += tmp;
+    acc2 += tmp2;
+  }
+  finish_block<false>(acc, acc2, nullptr, 0, R);
+}
+
+// ---------------------------------------------------------------------------
+// unfused path: sample (any n_dim)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void vegas_sample_f64(StepK s, const double* __restrict__ grid,
+                                                           double* __restrict__ xo,
+                                                           double* __restrict__ wo,
+                                                           uint8_t* __restrict__ io) {
+  const int d = s.n_dim;
+  const bool limits = s.xmin != nullptr;
+  const double* inj = static_cast<const double*>(s.inject);
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
+    const uint64_t ge = (uint64_t)(s.event_offset + i);
+    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    double w = 1.0;
+    for (int g = 0; g < s.groups; ++g) {
+      uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
+      if (!inj) {
+        h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+        l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = 4 * g + q;
+        if (j < d) {
+          const double u = inj ? inj[i * d + j] : draw_f64(word(h, q), word(l, q));
+          const double xn = kFBins * (1.0 - u);
+          const int ind = clamp_bin((int)xn);
+          const double lo = grid[j * (kBins + 1) + ind];
+          const double dx = grid[j * (kBins + 1) + ind + 1] - lo;
+          const double frac = xn - floor(xn);
+          double x = lo + dx * frac;  // two roundings, as vflow.py:76
+          w = w * (dx * kFBins);
+          if (limits) x = s.xmin[j] + x * s.xdelta[j];
+          xo[i * d + j] = x;
+          if (io) io[i * d + j] = (uint8_t)ind;
+        }
+      }
+    }
+    double wt = w * s.xjac;
+    if (limits) wt = wt * s.xdeltajac;
+    wo[i] = wt;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void vegas_sample_mixed(StepK s, const double* __restrict__ grid,
+                                                             float* __restrict__ xo,
+                                                             float* __restrict__ wo,
+                                                             uint8_t* __restrict__ io) {
+  const int d = s.n_dim;
+  const bool limits = s.xmin != nullptr;
+  const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
+  const float wscale = (float)(limits ? s.xjac * s.xdeltajac : s.xjac);
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
+    const uint64_t ge = (uint64_t)(s.event_offset + i);
+    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    float w = 1.0f;
+    for (int g = 0; g < s.groups; ++g) {
+      uint4 h = make_uint4(0, 0, 0, 0);
+      if (!inj) h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = 4 * g + q;
+        if (j < d) {
+          const uint32_t hw = inj ? inj[i * d + j] : word(h, q);
+          const uint64_t p = (uint64_t)(~hw) * (uint64_t)kBins;
+          const int ind = (int)(p >> 32);
+          const float frac = (float)(((uint32_t)p) >> 8) * 0x1p-24f;
+          const double lo = grid[j * (kBins + 1) + ind];
+          const double dx = grid[j * (kBins + 1) + ind + 1] - lo;
+          float x = fmaf((float)dx, frac, (float)lo);
+          w = w * (float)(dx * kFBins);
+          if (limits) x = fmaf(x, (float)s.xdelta[j], (float)s.xmin[j]);
+          xo[i * d + j] = x;
+          if (io) io[i * d + j] = (uint8_t)ind;
+        }
+      }
+    }
+    wo[i] = w * wscale;
+  }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void plain_sample(StepK s, void* __restrict__ xo_) {
+  const int d = s.n_dim;
+  const bool limits = s.xmin != nullptr;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
+    const uint64_t ge = (uint64_t)(s.event_offset + i);
+    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    for (int g = 0; g < s.groups; ++g) {
+      uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
+      if (!s.inject) {
+        h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+        if (!MIXED) l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = 4 * g + q;
+        if (j < d) {
+          if (MIXED) {
+            const uint32_t hw = s.inject ? static_cast<const uint32_t*>(s.inject)[i * d + j]
+                                         : word(h, q);
+            float x = fmaf((float)(hw >> 9), 0x1p-23f, 0x1p-24f);
+            if (limits) x = fmaf(x, (float)s.xdelta[j], (float)s.xmin[j]);
+            static_cast<float*>(xo_)[i * d + j] = x;
+          } else {
+            const double u = s.inject ? static_cast<const double*>(s.inject)[i * d + j]
+                                      : draw_f64(word(h, q), word(l, q));
+            static_cast<double*>(xo_)[i * d + j] = limits ? s.xmin[j] + u * s.xdelta[j] : u;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// unfused path: digest (vflow.py:414-430, plain.py:26-35)
+// grid.y splits the dimensions into slices of kDigestDims so that any n_dim
+// fits the per-wave LDS histograms; slice 0 also accumulates the sums.
+// ---------------------------------------------------------------------------
+constexpr int kDigestDims = MV_DIGEST_DIMS;
+
+template <typename FT, bool VEC, bool TRAIN>
+__global__ __launch_bounds__(kBlock) void vegas_digest(StepK s, const FT* __restrict__ f,
+                                                       int64_t fstride, int nout, int main_dim,
+                                                       const FT* __restrict__ w, int64_t wstride,
+                                                       const uint8_t* __restrict__ idx, RowsK R) {
+  __shared__ double hist[kWaves * kDigestDims * kBins];
+  __shared__ double red[kWaves * 2 * MV_MAX_OUT];
+  const int d = s.n_dim;
+  const int j0 = blockIdx.y * kDigestDims;
+  const int nd = min(kDigestDims, d - j0);
+  const int nb = nd * kBins;
+  if (TRAIN) {
+    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+    __syncthreads();
+  }
+  const int wave = threadIdx.x >> 6;
+  double* wh = hist + wave * nb;
+  const bool sums = blockIdx.y == 0;
+  constexpr int KM = VEC ? MV_MAX_OUT : 1;
+  double acc[KM], acc2[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) acc[k] = acc2[k] = 0.0;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
+    const double wi = (double)w[i * wstride];
+    double t2main;
+    if (VEC) {
+      t2main = 0.0;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        if (k < nout) {
+          const double tmp = wi * (double)f[i * fstride + k];  // vflow.py:416
+          const double tmp2 = tmp * tmp;
+          acc[k] += tmp;
+          acc2[k] += tmp2;
+          if (k == main_dim) t2main = tmp2;  // vflow.py:425-426
+        }
+      }
+    } else {
+      const double tmp = wi * (double)f[i * fstride];
+      t2main = tmp * tmp;
+      acc[0] += tmp;
+      acc2[0] += t2main;
+    }
+    if (TRAIN) {
+      const uint8_t* ir = idx + i * d + j0;
+      for (int j = 0; j < nd; ++j) lds_add(&wh[j * kBins + clamp_bin(ir[j])], t2main);
+    }
+  }
+  const int no = VEC ? nout : 1;
+  if (sums) {
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      if (k < no) {
+        const double a = wave_sum(acc[k]);
+        const double b = wave_sum(acc2[k]);
+        if ((threadIdx.x & 63) == 0) {
+          red[(2 * k) * kWaves + wave] = a;
+          red[(2 * k + 1) * kWaves + wave] = b;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double* row = R.blk + (int64_t)blockIdx.x * R.stride;
+  if (sums && threadIdx.x < 2 * no) {
+    const int v = threadIdx.x;  // 2k: sum of output k, 2k+1: its sum of squares
+    double a = red[v * kWaves];
+    for (int q = 1; q < kWaves; ++q) a += red[v * kWaves + q];
+    row[(v & 1) ? no + (v >> 1) : (v >> 1)] = a;
+  }
+  if (TRAIN) {
+    for (int i = threadIdx.x; i < nb; i += kBlock) {
+      double h = hist[i];
+      for (int q = 1; q < kWaves; ++q) h += hist[q * nb + i];
+      row[2 * no + j0 * kBins + i] = h;
+    }
+  }
+  // every y-slice folds its own columns; slice 0 owns the sums
+  group_fold_cols(R, sums ? 0 : 2 * no + j0 * kBins, 2 * no + j0 * kBins + (TRAIN ? nb : 0),
+                  blockIdx.y);
+}
+// ---------------------------------------------------------------------------
+// finalize: block 0 sums and sigma, block 1 + j dimension j (refine.hpp)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void finalize_kernel(FzK z, const double* __restrict__ P,
+                                                          int64_t stride, int nrows,
+                                                          double* __restrict__ grid,
+                                                          double* __restrict__ out,
+                                                          double* __restrict__ hist_out) {
+  if (blockIdx.x == 0)
+    finalize_sums_block(z, P, stride, nrows, out);
+  else
+    hist_refine_block(z, P, stride, nrows, (int)blockIdx.x - 1, grid, hist_out);
+}
+
+__global__ __launch_bounds__(64) void refine_kernel(const double* __restrict__ hist, int64_t hstride,
+                                                    double* __restrict__ grid) {
+  __shared__ double hb[kBins];
+  const int j = blockIdx.x, t = threadIdx.x;
+  if (t < kBins) hb[t] = hist[j * hstride + t];
+  __syncthreads();
+  refine_wave(hb, grid + j * (kBins + 1));
+}
+
+// ---------------------------------------------------------------------------
+// consume_array_into_indices (utils.py:10-44)
+// ---------------------------------------------------------------------------
+__global__ void fill_kernel(double* p, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+constexpr int kConsumeLds = 4096;
+
+__global__ __launch_bounds__(kBlock) void consume_kernel(const double* __restrict__ in,
+                                                         const int32_t* __restrict__ ind,
+                                                         int64_t n, int size,
+                                                         double* __restrict__ out) {
+  __shared__ double h[kConsumeLds];
+  const bool lds = size <= kConsumeLds;
+  if (lds) {
+    for (int i = threadIdx.x; i < size; i += kBlock) h[i] = 0.0;
+    __syncthreads();
+  }
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int b = ind[i];
+    if (b >= 0 && b < size) {
+      if (lds) lds_add(&h[b], in[i]);
+      else unsafeAtomicAdd(&out[b], in[i]);
+    }
+  }
+  if (lds) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < size; i += kBlock)
+      if (h[i] != 0.0) unsafeAtomicAdd(&out[i], h[i]);
+  }
+}
+
+}  // namespace mv
+
+using namespace mv;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int mv_abi_version(void) { return MV_ABI_VERSION; }
+
+extern "C" const char* mv_last_error(void) { return last_error_slot().c_str(); }
+
+extern "C" int64_t mv_struct_size(int32_t which) {
+  switch (which) {
+    case 0: return (int64_t)sizeof(mv_integrand);
+    case 1: return (int64_t)sizeof(mv_step);
+    case 2: return (int64_t)sizeof(mv_partials);
+    case 3: return (int64_t)sizeof(mv_strata);
+    case 4: return (int64_t)sizeof(mv_finalize);
+    default: return -1;
+  }
+}
+
+extern "C" int64_t mv_partial_stride(int32_t n_dim, int32_t n_out, int32_t train) {
+  return 2 * (int64_t)n_out + (train ? (int64_t)n_dim * kBins : 0);
+}
+
+extern "C" int32_t mv_producer_blocks(int64_t n_events) { return producer_blocks(n_events); }
+
+extern "C" int32_t mv_producer_groups(int64_t n_events) {
+  return groups_of(producer_blocks(n_events));
+}
+
+namespace {
+
+template <int FAM>
+void launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool train,
+                        const double* grid, const RowsK& R, int blocks, hipStream_t st) {
+  const int nb = k.n_dim * kBins;
+  if (!mixed) {
+    const size_t lds = (size_t)(3 * nb + (train ? kWaves * nb : 0)) * sizeof(double);
+    if (train)
+      hipLaunchKernelGGL((vegas_fused_f64<FAM, true>), dim3(blocks), dim3(kBlock), lds, st, k, fn, grid, R);
+    else
+      hipLaunchKernelGGL((vegas_fused_f64<FAM, false>), dim3(blocks), dim3(kBlock), lds, st, k, fn, grid, R);
+  } else {
+    const size_t lds = (size_t)(nb + (nb + 1) / 2 + (train ? kWaves * nb : 0)) * sizeof(double);
+    if (train)
+      hipLaunchKernelGGL((vegas_fused_mixed<FAM, true>), dim3(blocks), dim3(kBlock), lds, st, k, fn, grid, R);
+    else
+      hipLaunchKernelGGL((vegas_fused_mixed<FAM, false>), dim3(blocks), dim3(kBlock), lds, st, k, fn, grid, R);
+  }
+}
+
+template <int FAM>
+void launch_plain_fused(const StepK& k, const mv_integrand& fn, bool mixed, const RowsK& R,
+                        int blocks, hipStream_t st) {
+  if (mixed)
+    hipLaunchKernelGGL((plain_fused<FAM, true>), dim3(blocks), dim3(kBlock), 0, st, k, fn, R);
+  else
+    hipLaunchKernelGGL((plain_fused<FAM, false>), dim3(blocks), dim3(kBlock), 0, st, k, fn, R);
+}
+
+int sample_blocks(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (int)b;
+}
+
+}  // namespace
+
+extern "C" int mv_vegas_fused(const mv_step* step, const mv_integrand* fn, const double* grid,
+                              const mv_partials* parts, void* stream) {
+  int rc = validate_step(step, "mv_vegas_fused", MV_FUSED_MAX_DIM);
+  if (rc) return rc;
+  if ((rc = validate_fn(fn, "mv_vegas_fused"))) return rc;
+  if (!grid) return fail(MV_ERR_ARG, "mv_vegas_fused: grid is required");
+  RowsK R;
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train), step->accumulate,
+                      "mv_vegas_fused", &R)))
+    return rc;
+  const int blocks = producer_blocks(step->n_events);
+  const StepK k = make_stepk(*step);
+  const bool mixed = step->precision == MV_MIXED, train = step->train != 0;
+  hipStream_t st = as_stream(stream);
+  switch (fn->family) {
+    case MV_FN_CONSTANT: launch_vegas_fused<MV_FN_CONSTANT>(k, *fn, mixed, train, grid, R, blocks, st); break;
+    case MV_FN_GAUSSIAN: launch_vegas_fused<MV_FN_GAUSSIAN>(k, *fn, mixed, train, grid, R, blocks, st); break;
+    case MV_FN_PRODUCT: launch_vegas_fused<MV_FN_PRODUCT>(k, *fn, mixed, train, grid, R, blocks, st); break;
+    case MV_FN_GAUSSMIX: launch_vegas_fused<MV_FN_GAUSSMIX>(k, *fn, mixed, train, grid, R, blocks, st); break;
+    case MV_FN_BWPOWER: launch_vegas_fused<MV_FN_BWPOWER>(k, *fn, mixed, train, grid, R, blocks, st); break;
+  }
+  if ((rc = check_launch("mv_vegas_fused"))) return rc;
+  return groups_of(blocks);
+}
+
+extern "C" int mv_plain_fused(const mv_step* step, const mv_integrand* fn, const mv_partials* parts,
+                              void* stream) {
+  int rc = validate_step(step, "mv_plain_fused", 0);
+  if (rc) return rc;
+  if ((rc = validate_fn(fn, "mv_plain_fused"))) return rc;
+  if ((fn->family == MV_FN_GAUSSIAN || fn->family == MV_FN_GAUSSMIX) &&
+      step->n_dim > MV_FUSED_MAX_DIM)
+    return fail(MV_ERR_UNSUPPORTED, "mv_plain_fused: n_dim %d exceeds %d for this family",
+                step->n_dim, MV_FUSED_MAX_DIM);
+  RowsK R;
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, 0), step->accumulate,
+                      "mv_plain_fused", &R)))
+    return rc;
+  const int blocks = producer_blocks(step->n_events);
+  const StepK k = make_stepk(*step);
+  const bool mixed = step->precision == MV_MIXED;
+  hipStream_t st = as_stream(stream);
+  switch (fn->family) {
+    case MV_FN_CONSTANT: launch_plain_fused<MV_FN_CONSTANT>(k, *fn, mixed, R, blocks, st); break;
+    case MV_FN_GAUSSIAN: launch_plain_fused<MV_FN_GAUSSIAN>(k, *fn, mixed, R, blocks, st); break;
+    case MV_FN_PRODUCT: launch_plain_fused<MV_FN_PRODUCT>(k, *fn, mixed, R, blocks, st); break;
+    case MV_FN_GAUSSMIX: launch_plain_fused<MV_FN_GAUSSMIX>(k, *fn, mixed, R, blocks, st); break;
+    case MV_FN_BWPOWER: launch_plain_fused<MV_FN_BWPOWER>(k, *fn, mixed, R, blocks, st); break;
+  }
+  if ((rc = check_launch("mv_plain_fused"))) return rc;
+  return groups_of(blocks);
+}
+
+extern "C" int mv_vegas_sample(const mv_step* step, const double* grid, void* x, void* w,
+                               uint8_t* idx, void* stream) {
+  int rc = validate_step(step, "mv_vegas_sample", 0);
+  if (rc) return rc;
+  if (!grid || !x || !w) return fail(MV_ERR_ARG, "mv_vegas_sample: grid, x and w are required");
+  if (step->n_events == 0) return MV_OK;
+  const StepK k = make_stepk(*step);
+  hipStream_t st = as_stream(stream);
+  const int blocks = sample_blocks(step->n_events);
+  if (step->precision == MV_F64)
+    hipLaunchKernelGGL(vegas_sample_f64, dim3(blocks), dim3(kBlock), 0, st, k, grid,
+                       static_cast<double*>(x), static_cast<double*>(w), idx);
+  else
+    hipLaunchKernelGGL(vegas_sample_mixed, dim3(blocks), dim3(kBlock), 0, st, k, grid,
+                       static_cast<float*>(x), static_cast<float*>(w), idx);
+  return check_launch("mv_vegas_sample");
+}
+
+extern "C" int mv_plain_sample(const mv_step* step, void* x, void* stream) {
+  int rc = validate_step(step, "mv_plain_sample", 0);
+  if (rc) return rc;
+  if (!x) return fail(MV_ERR_ARG, "mv_plain_sample: x is required");
+  if (step->n_events == 0) return MV_OK;
+  const StepK k = make_stepk(*step);
+  hipStream_t st = as_stream(stream);
+  const int blocks = sample_blocks(step->n_events);
+  if (step->precision == MV_F64)
+    hipLaunchKernelGGL(plain_sample<false>, dim3(blocks), dim3(kBlock), 0, st, k, x);
+  else
+    hipLaunchKernelGGL(plain_sample<true>, dim3(blocks), dim3(kBlock), 0, st, k, x);
+  return check_launch("mv_plain_sample");
+}
+
+extern "C" int mv_vegas_digest(const mv_step* step, const void* f, int64_t f_stride, int32_t n_out,
+                               int32_t main_dim, const void* w, int64_t w_stride,
+                               const uint8_t* idx, const mv_partials* parts, void* stream) {
+  int rc = validate_step(step, "mv_vegas_digest", 0);
+  if (rc) return rc;
+  if (!f || !w) return fail(MV_ERR_ARG, "mv_vegas_digest: f and w are required");
+  if (n_out < 1 || n_out > MV_MAX_OUT)
+    return fail(MV_ERR_UNSUPPORTED, "mv_vegas_digest: n_out must be 1..%d (got %d)", MV_MAX_OUT, n_out);
+  if (main_dim < 0 || main_dim >= n_out)
+    return fail(MV_ERR_ARG, "mv_vegas_digest: main_dim %d out of range", main_dim);
+  if (step->train && !idx) return fail(MV_ERR_ARG, "mv_vegas_digest: idx required when training");
+  if (f_stride < n_out) return fail(MV_ERR_ARG, "mv_vegas_digest: f_stride < n_out");
+  if (w_stride < 0) return fail(MV_ERR_ARG, "mv_vegas_digest: negative w_stride");
+  RowsK R;
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, n_out, step->train), step->accumulate,
+                      "mv_vegas_digest", &R)))
+    return rc;
+  const int blocks = producer_blocks(step->n_events);
+  const StepK k = make_stepk(*step);
+  hipStream_t st = as_stream(stream);
+  const int slices = step->train ? (step->n_dim + kDigestDims - 1) / kDigestDims : 1;
+  if (slices > MV_MAX_SLICES)
+    return fail(MV_ERR_UNSUPPORTED, "mv_vegas_digest: n_dim %d exceeds %d", step->n_dim,
+                MV_MAX_SLICES * kDigestDims);
+  const dim3 g(blocks, slices);
+  const bool vec = n_out > 1, train = step->train != 0;
+#define MV_DIGEST(FT, V, TR)                                                                   \
+  hipLaunchKernelGGL((vegas_digest<FT, V, TR>), g, dim3(kBlock), 0, st, k,                     \
+                     static_cast<const FT*>(f), f_stride, n_out, main_dim,                     \
+                     static_cast<const FT*>(w), w_stride, idx, R)
+  if (step->precision == MV_F64) {
+    if (vec) { if (train) MV_DIGEST(double, true, true); else MV_DIGEST(double, true, false); }
+    else { if (train) MV_DIGEST(double, false, true); else MV_DIGEST(double, false, false); }
+  } else {
+    if (vec) { if (train) MV_DIGEST(float, true, true); else MV_DIGEST(float, true, false); }
+    else { if (train) MV_DIGEST(float, false, true); else MV_DIGEST(float, false, false); }
+  }
+#undef MV_DIGEST
+  if ((rc = check_launch("mv_vegas_digest"))) return rc;
+  return groups_of(blocks);
+}
+
+extern "C" int mv_vegas_finalize(const mv_finalize* fz, const double* rows, int64_t row_stride,
+                                 int32_t n_rows, double* grid, double* out, double* hist_out,
+                                 void* stream) {
+  if (!fz) return fail(MV_ERR_ARG, "mv_vegas_finalize: fz is NULL");
+  if (fz->n_dim < 1 || fz->n_out < 1 || fz->n_out > MV_MAX_OUT)
+    return fail(MV_ERR_ARG, "mv_vegas_finalize: bad n_dim/n_out");
+  if (!rows || !out || n_rows < 1) return fail(MV_ERR_ARG, "mv_vegas_finalize: rows/out/n_rows");
+  if (row_stride < mv_partial_stride(fz->n_dim, fz->n_out, fz->train))
+    return fail(MV_ERR_ARG, "mv_vegas_finalize: row_stride too small");
+  if (fz->refine && (!fz->train || !grid))
+    return fail(MV_ERR_ARG, "mv_vegas_finalize: refine needs train rows and a grid");
+  if (hist_out && !fz->train) return fail(MV_ERR_ARG, "mv_vegas_finalize: no histogram without train");
+  FzK z{fz->n_dim, fz->n_out, fz->train, fz->refine, fz->raw, fz->n_events};
+  const int blocks = 1 + ((fz->train && (fz->refine || hist_out)) ? fz->n_dim : 0);
+  hipLaunchKernelGGL(finalize_kernel, dim3(blocks), dim3(kBlock), 0, as_stream(stream), z, rows,
+                     row_stride, n_rows, grid, out, hist_out);
+  return check_launch("mv_vegas_finalize");
+}
+
+extern "C" int mv_refine_grid(int32_t n_dim, const double* hist, int64_t hist_stride, double* grid,
+                              void* stream) {
+  if (n_dim < 1 || !hist || !grid || hist_stride < kBins)
+    return fail(MV_ERR_ARG, "mv_refine_grid: bad arguments");
+  hipLaunchKernelGGL(refine_kernel, dim3(n_dim), dim3(64), 0, as_stream(stream), hist, hist_stride, grid);
+  return check_launch("mv_refine_grid");
+}
+
+extern "C" int mv_consume_array_into_indices(const double* input, const int32_t* indices, int64_t n,
+                                             int32_t result_size, double* out, void* stream) {
+  if (result_size < 0 || n < 0) return fail(MV_ERR_ARG, "mv_consume_array_into_indices: bad sizes");
+  if (result_size > 0 && !out) return fail(MV_ERR_ARG, "mv_consume_array_into_indices: NULL out");
+  if (n > 0 && (!input || !indices)) return fail(MV_ERR_ARG, "mv_consume_array_into_indices: NULL input");
+  hipStream_t st = as_stream(stream);
+  if (result_size == 0) return MV_OK;
+  hipLaunchKernelGGL(fill_kernel, dim3((result_size + 255) / 256), dim3(256), 0, st, out,
+                     (int64_t)result_size, 0.0);
+  if (n > 0) {
+    const int blocks = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, 1024);
+    hipLaunchKernelGGL(consume_kernel, dim3(blocks), dim3(kBlock), 0, st, input, indices, n,
+                       result_size, out);
+  }
+  return check_launch("mv_consume_array_into_indices");
+}

@@ -1,0 +1,589 @@
+// vplus.hip - VegasFlowPlus (VEGAS+, importance + adaptive stratified
+// sampling, vflowplus.py) kernels for gfx950 and their C ABI.
+//
+// Events of an iteration are laid out cube after cube: cube c owns events
+// offsets[c] .. offsets[c+1]-1 (offsets = exclusive prefix sum of n_ev,
+// vflowplus.py:67). A wave handles 64 consecutive events, so the cube ids in
+// a wave are sorted; per-cube sums are a segmented wave scan followed by one
+// FP64 global atomic per (wave, cube) segment.
+#include <algorithm>
+#include <type_traits>
+
+#include "abi_util.hpp"
+#include "device.hpp"
+#include "refine.hpp"
+#include "rows.hpp"
+
+namespace mv {
+
+struct StrataK {
+  int n_strat;
+  int n_cubes;
+  const int32_t* n_ev;
+  const int32_t* off;
+};
+
+// largest c with off[c] <= e (off strictly increasing, off[0] = 0)
+__device__ __forceinline__ int find_cube(const int32_t* __restrict__ off, int n_cubes, int e) {
+  int lo = 0, hi = n_cubes;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// q = c / n, r = c % n for 0 <= c < 2^24 through an FP32 reciprocal and
+// one correction step
+__device__ __forceinline__ void divmod_small(int c, int n, float rn, int& q, int& r) {
+  q = (int)((float)c * rn);
+  r = c - q * n;
+  if (r < 0) {
+    --q;
+    r += n;
+  } else if (r >= n) {
+    ++q;
+    r -= n;
+  }
+}
+
+// hypercube coordinates, last dimension fastest (vflowplus.py:126-128)
+__device__ __forceinline__ void cube_digits(int cube, int d, int n_strat, float rn,
+                                            int (&h)[MV_FUSED_MAX_DIM]) {
+  int c = cube;
+#pragma unroll
+  for (int j = MV_FUSED_MAX_DIM - 1; j >= 0; --j) {
+    h[j] = 0;
+    if (j < d) {
+      int q, r;
+      divmod_small(c, n_strat, rn, q, r);
+      h[j] = r;
+      c = q;
+    }
+  }
+}
+
+// Segmented suffix sum over the wave's (sorted) cube ids; the first lane of
+// each segment adds the segment total to ress / ress2 (vflowplus.py:213-214).
+__device__ __forceinline__ void segment_add(int cube, double v, double v2, double* __restrict__ ress,
+                                            double* __restrict__ ress2) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double ov = __shfl_down(v, off, 64);
+    const double ov2 = __shfl_down(v2, off, 64);
+    const int oc = __shfl_down(cube, off, 64);
+    if (lane + off < 64 && oc == cube) {
+      v += ov;
+      v2 += ov2;
+    }
+  }
+  const int pc = __shfl_up(cube, 1, 64);
+  const bool head = lane == 0 || pc != cube;
+  if (head && cube >= 0) {
+    unsafeAtomicAdd(&ress[cube], v);
+    unsafeAtomicAdd(&ress2[cube], v2);
+  }
+}
+
+__device__ __forceinline__ int clamp_bin(int b) { return b < 0 ? 0 : (b > kBins - 1 ? kBins - 1 : b); }
+
+template <bool TRAIN>
+__device__ __forceinline__ void finish_block(double acc, double acc2, const double* hist,
+                                             int hist_len, const RowsK& R) {
+  __shared__ double red[2 * kWaves];
+  acc = wave_sum(acc);
+  acc2 = wave_sum(acc2);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wave] = acc;
+    red[kWaves + wave] = acc2;
+  }
+  __syncthreads();
+  double* row = R.blk + (int64_t)blockIdx.x * R.stride;
+  if (threadIdx.x == 0) {
+    double a = red[0], b = red[kWaves];
+    for (int k = 1; k < kWaves; ++k) {
+      a += red[k];
+      b += red[kWaves + k];
+    }
+    row[0] = a;
+    row[1] = b;
+  }
+  if (TRAIN) {
+    for (int i = threadIdx.x; i < hist_len; i += kBlock) {
+      double h = hist[i];
+      for (int k = 1; k < kWaves; ++k) h += hist[k * hist_len + i];
+      row[2 + i] = h;
+    }
+  }
+  group_fold(R, 2 + (TRAIN ? hist_len : 0));
+}
+
+// ---------------------------------------------------------------------------
+// fused VEGAS+ (vflowplus.py:37-80 + 187-220, monte_carlo.py:249-275)
+// FP64: xn = ((h + u) * 50) / n_strat with three roundings (vflowplus.py:72).
+// Mixed: the same index arithmetic in FP64 from u = hi * 2^-32, FP32 map and
+// integrand, FP64 sums.
+// ---------------------------------------------------------------------------
+template <int FAM, bool TRAIN, bool MIXED>
+__global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_integrand fn,
+                                                      const double* __restrict__ grid,
+                                                      double* __restrict__ ress,
+                                                      double* __restrict__ ress2, RowsK R) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  using T2 = typename std::conditional<MIXED, float2, double2>::type;
+  extern __shared__ double smem[];
+  const int d = s.n_dim;
+  const int nb = d * kBins;
+  T2* tab = reinterpret_cast<T2*>(smem);
+  T* tw = reinterpret_cast<T*>(smem + (MIXED ? nb : 2 * nb));
+  double* hist = smem + (MIXED ? nb + (nb + 1) / 2 : 3 * nb);
+  const bool limits = s.xmin != nullptr;
+  for (int i = threadIdx.x; i < nb; i += kBlock) {
+    const int j = i / kBins, b = i - j * kBins;
+    const double lo = grid[j * (kBins + 1) + b];
+    const double dx = grid[j * (kBins + 1) + b + 1] - lo;
+    double P = lo, Q = dx;
+    if (limits) {
+      P = s.xmin[j] + lo * s.xdelta[j];
+      Q = dx * s.xdelta[j];
+    }
+    Integrand<T, FAM>::fold(fn, j, P, Q);
+    T2 e;
+    e.x = (T)P;
+    e.y = (T)Q;
+    tab[i] = e;
+    tw[i] = (T)(dx * kFBins);
+  }
+  if (TRAIN)
+    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* wh = hist + wave * nb;
+
+  Integrand<T, FAM> F(fn);
+  const double wscale = limits ? s.xjac * s.xdeltajac : s.xjac;
+  const double fstrat = (double)st.n_strat;
+  const float rn = 1.0f / (float)st.n_strat;
+  const int total = st.off[st.n_cubes];
+  double acc = 0.0, acc2 = 0.0;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock + wave * 64; base < total; base += step) {
+    const int64_t i = base + lane;
+    int cube = -1;
+    double tmp = 0.0, tmp2 = 0.0;
+    if (i < total) {
+      cube = find_cube(st.off, st.n_cubes, (int)i);
+      int h[MV_FUSED_MAX_DIM];
+      cube_digits(cube, d, st.n_strat, rn, h);
+      const uint64_t ge = (uint64_t)(s.event_offset + i);
+      const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+      F.begin();
+      T w = (T)1;
+      uint32_t pk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (g < s.groups) {
+          uint4 hw4 = make_uint4(0, 0, 0, 0), lw4 = make_uint4(0, 0, 0, 0);
+          if (!s.inject) {
+            hw4 = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+            if (!MIXED) lw4 = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j = 4 * g + q;
+            if (j < d) {
+              double u;
+              if (MIXED) {
+                const uint32_t hw = s.inject ? static_cast<const uint32_t*>(s.inject)[i * d + j]
+                                             : word(hw4, q);
+                u = (double)hw * 0x1p-32;
+              } else {
+                u = s.inject ? static_cast<const double*>(s.inject)[i * d + j]
+                             : draw_f64(word(hw4, q), word(lw4, q));
+              }
+              const double xn = (((double)h[j] + u) * kFBins) / fstrat;  // vflowplus.py:72
+              const int ind = clamp_bin((int)xn);
+              const T frac = (T)(xn - floor(xn));
+              const T2 t = tab[j * kBins + ind];
+              F.add_folded(j, MIXED ? (T)fmaf((float)t.y, (float)frac, (float)t.x)
+                                    : (T)fma((double)t.y, (double)frac, (double)t.x));
+              w = w * tw[j * kBins + ind];
+              if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
+            }
+          }
+        }
+      }
+      // vflowplus.py:77, monte_carlo.py:270-274, vflowplus.py:209
+      const double wev = (double)w / (double)st.n_ev[cube];
+      tmp = (wev * wscale) * (double)F.value();
+      tmp2 = tmp * tmp;
+      acc += tmp;
+      acc2 += tmp2;
+      if (TRAIN) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          if (g < s.groups) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int j = 4 * g + q;
+              if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
+            }
+          }
+        }
+      }
+    }
+    segment_add(cube, tmp, tmp2, ress, ress2);
+  }
+  finish_block<TRAIN>(acc, acc2, hist, nb, R);
+}
+
+// ---------------------------------------------------------------------------
+// unfused VEGAS+: sample (vflowplus.py:37-80) and digest (209-218)
+// ---------------------------------------------------------------------------
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void vplus_sample(StepK s, StrataK st,
+                                                       const double* __restrict__ grid,
+                                                       void* __restrict__ xo_, void* __restrict__ wo_,
+                                                       uint8_t* __restrict__ io,
+                                                       int32_t* __restrict__ so) {
+  const int d = s.n_dim;
+  const bool limits = s.xmin != nullptr;
+  const double fstrat = (double)st.n_strat;
+  const float rn = 1.0f / (float)st.n_strat;
+  const int total = st.off[st.n_cubes];
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += step) {
+    const int cube = find_cube(st.off, st.n_cubes, (int)i);
+    const uint64_t ge = (uint64_t)(s.event_offset + i);
+    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    // digits from the last dimension backwards
+    int rest = cube;
+    int hcache[4];
+    double w = 1.0;
+    float wf = 1.0f;
+    for (int g = s.groups - 1; g >= 0; --g) {
+      for (int q = 3; q >= 0; --q) {
+        const int j = 4 * g + q;
+        hcache[q] = 0;
+        if (j < d) {
+          int qq, r;
+          divmod_small(rest, st.n_strat, rn, qq, r);
+          hcache[q] = r;
+          rest = qq;
+        }
+      }
+      uint4 hw4 = make_uint4(0, 0, 0, 0), lw4 = make_uint4(0, 0, 0, 0);
+      if (!s.inject) {
+        hw4 = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+        if (!MIXED) lw4 = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = 4 * g + q;
+        if (j < d) {
+          double u;
+          if (MIXED) {
+            const uint32_t hw = s.inject ? static_cast<const uint32_t*>(s.inject)[i * d + j]
+                                         : word(hw4, q);
+            u = (double)hw * 0x1p-32;
+          } else {
+            u = s.inject ? static_cast<const double*>(s.inject)[i * d + j]
+                         : draw_f64(word(hw4, q), word(lw4, q));
+          }
+          const double xn = (((double)hcache[q] + u) * kFBins) / fstrat;
+          const int ind = clamp_bin((int)xn);
+          const double lo = grid[j * (kBins + 1) + ind];
+          const double dx = grid[j * (kBins + 1) + ind + 1] - lo;
+          const double frac = xn - floor(xn);
+          if (MIXED) {
+            float x = fmaf((float)dx, (float)frac, (float)lo);
+            wf = wf * (float)(dx * kFBins);
+            if (limits) x = fmaf(x, (float)s.xdelta[j], (float)s.xmin[j]);
+            static_cast<float*>(xo_)[i * d + j] = x;
+          } else {
+            double x = lo + dx * frac;  // vflow.py:76
+            w = w * (dx * kFBins);      // vflow.py:78
+            if (limits) x = s.xmin[j] + x * s.xdelta[j];
+            static_cast<double*>(xo_)[i * d + j] = x;
+          }
+          if (io) io[i * d + j] = (uint8_t)ind;
+        }
+      }
+    }
+    if (MIXED) {
+      static_cast<float*>(wo_)[i] =
+          (float)(((double)wf / (double)st.n_ev[cube]) * (limits ? s.xjac * s.xdeltajac : s.xjac));
+    } else {
+      double wt = (w / (double)st.n_ev[cube]) * s.xjac;  // vflowplus.py:77, monte_carlo.py:270
+      if (limits) wt = wt * s.xdeltajac;
+      static_cast<double*>(wo_)[i] = wt;
+    }
+    if (so) so[i] = cube;
+  }
+}
+
+template <typename FT, bool TRAIN>
+__global__ __launch_bounds__(kBlock) void vplus_digest(StepK s, StrataK st, const FT* __restrict__ f,
+                                                       const FT* __restrict__ w,
+                                                       const uint8_t* __restrict__ idx,
+                                                       const int32_t* __restrict__ seg,
+                                                       double* __restrict__ ress,
+                                                       double* __restrict__ ress2, RowsK R) {
+  extern __shared__ double hist[];
+  const int d = s.n_dim;
+  const int nb = d * kBins;
+  if (TRAIN)
+    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* wh = hist + wave * nb;
+  const int total = st.off[st.n_cubes];
+  double acc = 0.0, acc2 = 0.0;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock + wave * 64; base < total; base += step) {
+    const int64_t i = base + lane;
+    int cube = -1;
+    double tmp = 0.0, tmp2 = 0.0;
+    if (i < total) {
+      cube = seg[i];
+      tmp = (double)w[i] * (double)f[i];  // vflowplus.py:209
+      tmp2 = tmp * tmp;                   // vflowplus.py:210
+      acc += tmp;
+      acc2 += tmp2;
+      if (TRAIN)
+        for (int j = 0; j < d; ++j) lds_add(&wh[j * kBins + clamp_bin(idx[i * d + j])], tmp2);
+    }
+    segment_add(cube, tmp, tmp2, ress, ress2);
+  }
+  finish_block<TRAIN>(acc, acc2, hist, nb, R);
+}
+
+// ---------------------------------------------------------------------------
+// iteration end over the cubes (vflowplus.py:222-242 + 153-163), one
+// workgroup of 1024 threads; thread t owns a contiguous chunk of cubes.
+// ---------------------------------------------------------------------------
+constexpr int kCubeThreads = 1024;
+
+__device__ __forceinline__ double block_sum_1024(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = kCubeThreads / 2; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kCubeThreads) void vplus_cubes_kernel(
+    int n_cubes, int32_t* __restrict__ n_ev, int32_t* __restrict__ off, double* __restrict__ ress,
+    double* __restrict__ ress2, int adaptive, int min_neval, double init_calls, double beta,
+    double* __restrict__ out, double* __restrict__ arr_var_out) {
+  __shared__ double red[kCubeThreads];
+  __shared__ int iscan[kCubeThreads];
+  const int t = threadIdx.x;
+  const int per = (n_cubes + kCubeThreads - 1) / kCubeThreads;
+  const int c0 = min(n_cubes, t * per), c1 = min(n_cubes, c0 + per);
+  double rs = 0.0, s2 = 0.0;
+  for (int c = c0; c < c1; ++c) {
+    const double fn = (double)n_ev[c];
+    const double r = ress[c];
+    const double var = ress2[c] * fn - r * r;  // vflowplus.py:216-217
+    if (arr_var_out) arr_var_out[c] = var;
+    rs += r;                                   // vflowplus.py:231
+    s2 += fmax(var, 0.0) / (fn - 1.0);         // vflowplus.py:230, 232
+  }
+  const double res = block_sum_1024(rs, red);
+  const double sig2 = block_sum_1024(s2, red);
+  if (adaptive) {
+    // damped = arr_var^(beta/2) (vflowplus.py:157), clamped at zero first
+    double dl = 0.0;
+    for (int c = c0; c < c1; ++c) {
+      const double fn = (double)n_ev[c];
+      const double r = ress[c];
+      dl += pow(fmax(ress2[c] * fn - r * r, 0.0), 0.5 * beta);
+    }
+    const double dsum = block_sum_1024(dl, red);
+    if (dsum > 0.0) {
+      for (int c = c0; c < c1; ++c) {
+        const double fn = (double)n_ev[c];
+        const double r = ress[c];
+        const double damped = pow(fmax(ress2[c] * fn - r * r, 0.0), 0.5 * beta);
+        const double nv = fmax((double)min_neval, damped * init_calls / 2.0 / dsum);  // 158-161
+        n_ev[c] = (int32_t)nv;                                                           // 162
+      }
+    }
+  }
+  // offsets = exclusive prefix sum of n_ev (vflowplus.py:67)
+  int local = 0;
+  for (int c = c0; c < c1; ++c) local += n_ev[c];
+  iscan[t] = local;
+  __syncthreads();
+  for (int o = 1; o < kCubeThreads; o <<= 1) {
+    const int v = t >= o ? iscan[t - o] : 0;
+    __syncthreads();
+    iscan[t] += v;
+    __syncthreads();
+  }
+  int run = iscan[t] - local;
+  for (int c = c0; c < c1; ++c) {
+    off[c] = run;
+    run += n_ev[c];
+    ress[c] = 0.0;
+    ress2[c] = 0.0;
+  }
+  if (t == kCubeThreads - 1) {
+    off[n_cubes] = iscan[t];
+    out[0] = res;
+    out[1] = sqrt(sig2);  // vflowplus.py:233
+    out[2] = (double)iscan[t];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void vplus_hist_kernel(FzK z, const double* __restrict__ P,
+                                                            int64_t stride, int nrows,
+                                                            double* __restrict__ grid,
+                                                            double* __restrict__ hist_out) {
+  hist_refine_block(z, P, stride, nrows, (int)blockIdx.x, grid, hist_out);
+}
+
+}  // namespace mv
+
+using namespace mv;
+
+namespace {
+
+int validate_strata(const mv_strata* st, int n_dim, const char* what) {
+  if (!st || !st->n_ev || !st->offsets) return fail(MV_ERR_ARG, "%s: strata incomplete", what);
+  if (st->n_strat < 1 || st->n_cubes < 1) return fail(MV_ERR_ARG, "%s: bad stratification", what);
+  if (st->n_cubes >= (1 << 24)) return fail(MV_ERR_UNSUPPORTED, "%s: too many hypercubes", what);
+  int64_t p = 1;
+  for (int j = 0; j < n_dim && p <= st->n_cubes; ++j) p *= st->n_strat;
+  if (p != st->n_cubes) return fail(MV_ERR_ARG, "%s: n_cubes != n_strat^n_dim", what);
+  return MV_OK;
+}
+
+StrataK make_strata(const mv_strata& s) { return StrataK{s.n_strat, s.n_cubes, s.n_ev, s.offsets}; }
+
+template <int FAM>
+void launch_vplus_fused(const StepK& k, const StrataK& st, const mv_integrand& fn, bool mixed,
+                        bool train, const double* grid, double* ress, double* ress2, const RowsK& R,
+                        int blocks, hipStream_t s) {
+  const int nb = k.n_dim * kBins;
+  const size_t hist = train ? (size_t)kWaves * nb : 0;
+  const size_t lds = ((mixed ? nb + (nb + 1) / 2 : 3 * nb) + hist) * sizeof(double);
+#define MV_VPF(TR, MX) \
+  hipLaunchKernelGGL((vplus_fused<FAM, TR, MX>), dim3(blocks), dim3(kBlock), lds, s, k, st, fn, grid, ress, ress2, R)
+  if (mixed) { if (train) MV_VPF(true, true); else MV_VPF(false, true); }
+  else { if (train) MV_VPF(true, false); else MV_VPF(false, false); }
+#undef MV_VPF
+}
+
+}  // namespace
+
+extern "C" int mv_vplus_fused(const mv_step* step, const mv_strata* strata, const mv_integrand* fn,
+                              const double* grid, double* ress, double* ress2,
+                              const mv_partials* parts, void* stream) {
+  int rc = validate_step(step, "mv_vplus_fused", MV_FUSED_MAX_DIM);
+  if (rc) return rc;
+  if ((rc = validate_fn(fn, "mv_vplus_fused"))) return rc;
+  if ((rc = validate_strata(strata, step->n_dim, "mv_vplus_fused"))) return rc;
+  if (!grid || !ress || !ress2) return fail(MV_ERR_ARG, "mv_vplus_fused: grid, ress, ress2 required");
+  if (step->n_events >= (int64_t)1 << 31) return fail(MV_ERR_UNSUPPORTED, "mv_vplus_fused: too many events");
+  RowsK R;
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train), step->accumulate,
+                      "mv_vplus_fused", &R)))
+    return rc;
+  const int blocks = producer_blocks(step->n_events);
+  const StepK k = make_stepk(*step);
+  const StrataK st = make_strata(*strata);
+  const bool mixed = step->precision == MV_MIXED, train = step->train != 0;
+  hipStream_t s = as_stream(stream);
+  switch (fn->family) {
+    case MV_FN_CONSTANT: launch_vplus_fused<MV_FN_CONSTANT>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
+    case MV_FN_GAUSSIAN: launch_vplus_fused<MV_FN_GAUSSIAN>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
+    case MV_FN_PRODUCT: launch_vplus_fused<MV_FN_PRODUCT>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
+    case MV_FN_GAUSSMIX: launch_vplus_fused<MV_FN_GAUSSMIX>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
+    case MV_FN_BWPOWER: launch_vplus_fused<MV_FN_BWPOWER>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
+  }
+  if ((rc = check_launch("mv_vplus_fused"))) return rc;
+  return groups_of(blocks);
+}
+
+extern "C" int mv_vplus_sample(const mv_step* step, const mv_strata* strata, const double* grid,
+                               void* x, void* w, uint8_t* idx, int32_t* seg, void* stream) {
+  int rc = validate_step(step, "mv_vplus_sample", 0);
+  if (rc) return rc;
+  if ((rc = validate_strata(strata, step->n_dim, "mv_vplus_sample"))) return rc;
+  if (!grid || !x || !w) return fail(MV_ERR_ARG, "mv_vplus_sample: grid, x and w are required");
+  if (step->n_events >= (int64_t)1 << 31) return fail(MV_ERR_UNSUPPORTED, "mv_vplus_sample: too many events");
+  const StepK k = make_stepk(*step);
+  const StrataK st = make_strata(*strata);
+  int64_t b = (step->n_events + kBlock - 1) / kBlock;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(b, 8192));
+  if (step->precision == MV_MIXED)
+    hipLaunchKernelGGL(vplus_sample<true>, dim3(blocks), dim3(kBlock), 0, as_stream(stream), k, st, grid, x, w, idx, seg);
+  else
+    hipLaunchKernelGGL(vplus_sample<false>, dim3(blocks), dim3(kBlock), 0, as_stream(stream), k, st, grid, x, w, idx, seg);
+  return check_launch("mv_vplus_sample");
+}
+
+extern "C" int mv_vplus_digest(const mv_step* step, const mv_strata* strata, const void* f,
+                               const void* w, const uint8_t* idx, const int32_t* seg, double* ress,
+                               double* ress2, const mv_partials* parts, void* stream) {
+  int rc = validate_step(step, "mv_vplus_digest", 0);
+  if (rc) return rc;
+  if ((rc = validate_strata(strata, step->n_dim, "mv_vplus_digest"))) return rc;
+  if (!f || !w || !seg || !ress || !ress2) return fail(MV_ERR_ARG, "mv_vplus_digest: missing buffer");
+  if (step->train && !idx) return fail(MV_ERR_ARG, "mv_vplus_digest: idx required when training");
+  if (step->train && step->n_dim > 40)
+    return fail(MV_ERR_UNSUPPORTED, "mv_vplus_digest: n_dim > 40 with training");
+  RowsK R;
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train), step->accumulate,
+                      "mv_vplus_digest", &R)))
+    return rc;
+  const int blocks = producer_blocks(step->n_events);
+  const StepK k = make_stepk(*step);
+  const StrataK st = make_strata(*strata);
+  const size_t lds = step->train ? (size_t)kWaves * step->n_dim * kBins * sizeof(double) : 0;
+  hipStream_t s = as_stream(stream);
+#define MV_VPD(FT, TR) \
+  hipLaunchKernelGGL((vplus_digest<FT, TR>), dim3(blocks), dim3(kBlock), lds, s, k, st, \
+                     static_cast<const FT*>(f), static_cast<const FT*>(w), idx, seg, ress, ress2, R)
+  if (step->precision == MV_MIXED) { if (step->train) MV_VPD(float, true); else MV_VPD(float, false); }
+  else { if (step->train) MV_VPD(double, true); else MV_VPD(double, false); }
+#undef MV_VPD
+  if ((rc = check_launch("mv_vplus_digest"))) return rc;
+  return groups_of(blocks);
+}
+
+extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t* n_ev, int32_t* offsets,
+                                 double* ress, double* ress2, int32_t adaptive, int32_t min_neval,
+                                 double init_calls, double beta, const double* rows, int64_t row_stride,
+                                 int32_t n_rows, int32_t train, int32_t refine, double* grid,
+                                 double* out, double* arr_var_out, double* hist_out, void* stream) {
+  int rc = validate_strata(strata, n_dim, "mv_vplus_finalize");
+  if (rc) return rc;
+  if (!n_ev || !offsets || !ress || !ress2 || !out)
+    return fail(MV_ERR_ARG, "mv_vplus_finalize: missing buffer");
+  if (adaptive && min_neval < 1) return fail(MV_ERR_ARG, "mv_vplus_finalize: min_neval < 1");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(vplus_cubes_kernel, dim3(1), dim3(kCubeThreads), 0, s, strata->n_cubes, n_ev,
+                     offsets, ress, ress2, adaptive, min_neval, init_calls, beta, out, arr_var_out);
+  if ((rc = check_launch("mv_vplus_finalize"))) return rc;
+  if (train && (refine || hist_out)) {
+    if (!rows || n_rows < 1) return fail(MV_ERR_ARG, "mv_vplus_finalize: histogram rows required");
+    if (row_stride < mv_partial_stride(n_dim, 1, 1))
+      return fail(MV_ERR_ARG, "mv_vplus_finalize: row_stride too small");
+    if (refine && !grid) return fail(MV_ERR_ARG, "mv_vplus_finalize: grid required to refine");
+    FzK z{n_dim, 1, 1, refine, 0, 0.0};
+    hipLaunchKernelGGL(vplus_hist_kernel, dim3(n_dim), dim3(kBlock), 0, s, z, rows, row_stride, n_rows,
+                       grid, hist_out);
+    if ((rc = check_launch("mv_vplus_finalize"))) return rc;
+  }
+  return MV_OK;
+}

@@ -1,0 +1,278 @@
+"""Torch-tensor front end of libmivegas.so.
+
+One function per C entry point of include/mivegas.h. Each checks device,
+dtype and contiguity of its tensors, fills the ABI structs and enqueues the
+kernels on torch's current HIP stream; nothing here synchronises. The
+integrator classes are built on these functions, and the parity tests call
+them directly (e.g. with injected draws).
+"""
+import ctypes
+
+import torch
+
+from . import _abi
+
+BINS = _abi.BINS
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _need_device(t, name, dtype=None):
+    if t is None:
+        return
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a HIP device tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "vegasflow_amd runs its kernels on a ROCm GPU and none is available "
+            "(torch.cuda.is_available() is False)"
+        )
+
+
+def precision_code(precision):
+    if precision in ("f64", _abi.F64):
+        return _abi.F64
+    if precision in ("mixed", _abi.MIXED):
+        return _abi.MIXED
+    raise ValueError(f"unknown precision {precision!r}")
+
+
+def float_dtype(precision):
+    return torch.float64 if precision_code(precision) == _abi.F64 else torch.float32
+
+
+class Workspace:
+    """Device buffers for the partial rows of producer launches (caller-owned
+    memory of the ABI). Tickets start at zero and every launch leaves them
+    at zero."""
+
+    def __init__(self, device, stride):
+        self.device = device
+        self.stride = int(stride)
+        self.block_rows = torch.empty(_abi.MAX_BLOCKS * self.stride, dtype=torch.float64, device=device)
+        self.group_rows = torch.zeros(_abi.MAX_GROUPS * self.stride, dtype=torch.float64, device=device)
+        self.tickets = torch.zeros(_abi.TICKETS, dtype=torch.int32, device=device)
+        self._struct = _abi.Partials(
+            self.block_rows.data_ptr(), self.group_rows.data_ptr(), self.tickets.data_ptr(), self.stride
+        )
+
+    def partials(self):
+        return ctypes.byref(self._struct)
+
+
+_WS = {}
+
+
+def workspace(device, stride):
+    """A cached workspace with at least ``stride`` doubles per row (shared
+    by all stream-ordered calls on ``device``)."""
+    key = (str(device),)
+    ws = _WS.get(key)
+    if ws is None or ws.stride < stride:
+        ws = Workspace(device, max(stride, 64))
+        _WS[key] = ws
+    return ws
+
+
+def partial_stride(n_dim, n_out=1, train=True):
+    return 2 * n_out + (n_dim * BINS if train else 0)
+
+
+def producer_groups(n_events):
+    return _abi.load().mv_producer_groups(int(n_events))
+
+
+def make_step(n_dim, n_events, *, precision="f64", train=True, accumulate=False, event_offset=0,
+              seed=0, draw_id=0, xjac=1.0, xmin=None, xdelta=None, xdeltajac=1.0, inject=None):
+    _need_device(xmin, "xmin", torch.float64)
+    _need_device(xdelta, "xdelta", torch.float64)
+    if inject is not None:
+        want = torch.float64 if precision_code(precision) == _abi.F64 else torch.int32
+        _need_device(inject, "inject", want if want == torch.float64 else None)
+        if want != torch.float64 and inject.dtype not in (torch.int32, torch.uint32):
+            raise ValueError("mixed-precision inject takes the 32-bit words as int32/uint32")
+        if tuple(inject.shape) != (int(n_events), int(n_dim)):
+            raise ValueError(f"inject must have shape ({n_events}, {n_dim})")
+    s = _abi.Step()
+    s.n_dim = int(n_dim)
+    s.precision = precision_code(precision)
+    s.train = int(bool(train))
+    s.accumulate = int(bool(accumulate))
+    s.n_events = int(n_events)
+    s.event_offset = int(event_offset)
+    s.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    s.draw_id = int(draw_id) & 0xFFFFFFFF
+    s.xjac = float(xjac)
+    s.xdeltajac = float(xdeltajac)
+    s.xmin = None if xmin is None else xmin.data_ptr()
+    s.xdelta = None if xdelta is None else xdelta.data_ptr()
+    s.inject = None if inject is None else inject.data_ptr()
+    return s
+
+
+# ---------------------------------------------------------------------------
+# VegasFlow / PlainFlow
+# ---------------------------------------------------------------------------
+def vegas_fused(step, fn_struct, grid, ws):
+    """mv_vegas_fused; returns the number of group rows in ``ws``."""
+    _need_device(grid, "grid", torch.float64)
+    return _abi.call("mv_vegas_fused", ctypes.byref(step), ctypes.byref(fn_struct), _ptr(grid),
+                     ws.partials(), _stream())
+
+
+def plain_fused(step, fn_struct, ws):
+    return _abi.call("mv_plain_fused", ctypes.byref(step), ctypes.byref(fn_struct), ws.partials(),
+                     _stream())
+
+
+def vegas_sample(step, grid, x, w, idx=None):
+    _need_device(grid, "grid", torch.float64)
+    ft = float_dtype(step.precision)
+    _need_device(x, "x", ft)
+    _need_device(w, "w", ft)
+    _need_device(idx, "idx", torch.uint8)
+    n, d = step.n_events, step.n_dim
+    if x.numel() < n * d or w.numel() < n or (idx is not None and idx.numel() < n * d):
+        raise ValueError("sample buffers are too small")
+    _abi.call("mv_vegas_sample", ctypes.byref(step), _ptr(grid), _ptr(x), _ptr(w), _ptr(idx), _stream())
+
+
+def plain_sample(step, x):
+    _need_device(x, "x", float_dtype(step.precision))
+    if x.numel() < step.n_events * step.n_dim:
+        raise ValueError("sample buffer is too small")
+    _abi.call("mv_plain_sample", ctypes.byref(step), _ptr(x), _stream())
+
+
+def vegas_digest(step, f, w, idx, ws, main_dim=0):
+    """mv_vegas_digest for f of shape (n,) or (n, k); w of shape (n,) or (1,)."""
+    ft = float_dtype(step.precision)
+    _need_device(f, "f", ft)
+    _need_device(w, "w", ft)
+    _need_device(idx, "idx", torch.uint8)
+    n = step.n_events
+    if f.dim() == 1:
+        n_out, f_stride = 1, 1
+    else:
+        n_out, f_stride = f.shape[1], f.stride(0)
+    if f.shape[0] != n:
+        raise ValueError(f"integrand output has {f.shape[0]} rows, expected {n}")
+    w_stride = 0 if w.numel() == 1 else 1
+    if w_stride and w.numel() != n:
+        raise ValueError("w must have one entry per event or a single entry")
+    return _abi.call("mv_vegas_digest", ctypes.byref(step), _ptr(f), f_stride, n_out, int(main_dim),
+                     _ptr(w), w_stride, _ptr(idx), ws.partials(), _stream())
+
+
+def vegas_finalize(n_dim, n_out, train, refine, n_events, ws, n_rows, grid=None, hist_out=None,
+                   raw=False):
+    """Reduce group rows; returns a (2 * n_out,) float64 tensor [res, sigma]
+    (or [res, sum tmp^2] with raw)."""
+    out = torch.empty(2 * n_out, dtype=torch.float64, device=ws.device)
+    _need_device(grid, "grid", torch.float64)
+    _need_device(hist_out, "hist_out", torch.float64)
+    fz = _abi.Finalize()
+    fz.n_dim = int(n_dim)
+    fz.n_out = int(n_out)
+    fz.train = int(bool(train))
+    fz.refine = int(bool(refine))
+    fz.raw = int(bool(raw))
+    fz.n_events = float(n_events)
+    _abi.call("mv_vegas_finalize", ctypes.byref(fz), _ptr(ws.group_rows), ws.stride, int(n_rows),
+              _ptr(grid), _ptr(out), _ptr(hist_out), _stream())
+    return out
+
+
+def refine_grid(hist, grid):
+    """mv_refine_grid: hist (n_dim, 50) float64, grid (n_dim, 51) in place."""
+    _need_device(grid, "grid", torch.float64)
+    if hist.dtype != torch.float64:
+        hist = hist.to(torch.float64)
+    hist = hist.contiguous()
+    _need_device(hist, "hist", torch.float64)
+    if grid.dim() != 2 or grid.shape[1] != BINS + 1 or hist.shape != (grid.shape[0], BINS):
+        raise ValueError("refine_grid expects hist (n_dim, 50) and grid (n_dim, 51)")
+    _abi.call("mv_refine_grid", grid.shape[0], _ptr(hist), BINS, _ptr(grid), _stream())
+
+
+def consume_array_into_indices(input_arr, indices, result_size):
+    """mv_consume_array_into_indices (utils.py:10-44) on device tensors."""
+    inp = input_arr.to(torch.float64).contiguous().reshape(-1)
+    ind = indices.to(torch.int32).contiguous().reshape(-1)
+    _need_device(inp, "input_arr", torch.float64)
+    _need_device(ind, "indices", torch.int32)
+    if inp.numel() != ind.numel():
+        raise ValueError("input_arr and indices must have the same number of entries")
+    out = torch.empty(int(result_size), dtype=torch.float64, device=inp.device)
+    _abi.call("mv_consume_array_into_indices", _ptr(inp), _ptr(ind), inp.numel(), int(result_size),
+              _ptr(out), _stream())
+    return out
+
+
+# ---------------------------------------------------------------------------
+# VegasFlowPlus
+# ---------------------------------------------------------------------------
+def make_strata(n_strat, n_cubes, n_ev, offsets):
+    _need_device(n_ev, "n_ev", torch.int32)
+    _need_device(offsets, "offsets", torch.int32)
+    st = _abi.Strata()
+    st.n_strat = int(n_strat)
+    st.n_cubes = int(n_cubes)
+    st.n_ev = n_ev.data_ptr()
+    st.offsets = offsets.data_ptr()
+    return st
+
+
+def vplus_fused(step, strata, fn_struct, grid, ress, ress2, ws):
+    _need_device(grid, "grid", torch.float64)
+    _need_device(ress, "ress", torch.float64)
+    _need_device(ress2, "ress2", torch.float64)
+    return _abi.call("mv_vplus_fused", ctypes.byref(step), ctypes.byref(strata), ctypes.byref(fn_struct),
+                     _ptr(grid), _ptr(ress), _ptr(ress2), ws.partials(), _stream())
+
+
+def vplus_sample(step, strata, grid, x, w, idx, seg):
+    ft = float_dtype(step.precision)
+    _need_device(grid, "grid", torch.float64)
+    _need_device(x, "x", ft)
+    _need_device(w, "w", ft)
+    _need_device(idx, "idx", torch.uint8)
+    _need_device(seg, "seg", torch.int32)
+    _abi.call("mv_vplus_sample", ctypes.byref(step), ctypes.byref(strata), _ptr(grid), _ptr(x), _ptr(w),
+              _ptr(idx), _ptr(seg), _stream())
+
+
+def vplus_digest(step, strata, f, w, idx, seg, ress, ress2, ws):
+    ft = float_dtype(step.precision)
+    _need_device(f, "f", ft)
+    _need_device(w, "w", ft)
+    _need_device(idx, "idx", torch.uint8)
+    _need_device(seg, "seg", torch.int32)
+    if f.dim() != 1:
+        raise NotImplementedError("VegasFlowPlus does not support vectorial integrands")
+    return _abi.call("mv_vplus_digest", ctypes.byref(step), ctypes.byref(strata), _ptr(f), _ptr(w),
+                     _ptr(idx), _ptr(seg), _ptr(ress), _ptr(ress2), ws.partials(), _stream())
+
+
+def vplus_finalize(n_dim, strata, n_ev, offsets, ress, ress2, adaptive, min_neval, init_calls, beta,
+                   ws, n_rows, train, refine, grid, arr_var_out=None, hist_out=None):
+    """Returns a (3,) float64 tensor: res, sigma, events of the next iteration."""
+    out = torch.empty(3, dtype=torch.float64, device=ws.device)
+    _abi.call("mv_vplus_finalize", int(n_dim), ctypes.byref(strata), _ptr(n_ev), _ptr(offsets), _ptr(ress),
+              _ptr(ress2), int(bool(adaptive)), int(min_neval), float(init_calls), float(beta),
+              _ptr(ws.group_rows), ws.stride, int(n_rows), int(bool(train)), int(bool(refine)),
+              _ptr(grid), _ptr(out), _ptr(arr_var_out), _ptr(hist_out), _stream())
+    return out
