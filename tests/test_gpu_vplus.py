@@ -1,0 +1,180 @@
+"""VegasFlowPlus kernels (mv_vplus_*) against the oracle's restatement of
+vflowplus.py on the same draws.
+
+Cube layout, per-cube sums (segmented wave scans + atomics), sigma over the
+cubes, the adaptive reallocation of events (beta = 0.75) and the grid
+refinement are compared with oracle/vegas.py. Bins and cube ids are
+bit-exact; sums and edges within 1e-12 relative; the reallocated event counts
+come from int() of a float whose denominator is summed in a different order,
+so they must match exactly except for rare +-1 differences.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import vegas as ora
+from vegasflow_amd import kernels
+from vegasflow_amd import integrands as vfi
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+def random_grid(rng, d):
+    g = np.empty((d, 51))
+    for j in range(d):
+        w = rng.gamma(0.7, size=50) + 1e-3
+        c = np.concatenate([[0.0], np.cumsum(w)])
+        g[j] = c / c[-1]
+        g[j, 0], g[j, -1] = 0.0, 1.0
+    return g
+
+
+MIX = vfi.GaussianMix((0.5, 0.5), (0.05, 0.1), (0.33, 0.67))
+
+
+def mix_np(x):
+    return ora.gaussian_mix(x, (0.5, 0.5), (0.05, 0.1), (0.33, 0.67))
+
+
+def _setup(d, n, adaptive, rng, ragged=True):
+    st = ora.vplus_setup(d, n, adaptive)
+    if ragged:
+        st["n_ev"] = rng.integers(st["min_neval"], st["min_neval"] + 40, size=st["n_cubes"]).astype(np.int32)
+    return st
+
+
+def _device_strata(st):
+    n_ev = dev(st["n_ev"], torch.int32)
+    off = np.zeros(st["n_cubes"] + 1, dtype=np.int64)
+    off[1:] = np.cumsum(st["n_ev"])
+    offsets = dev(off, torch.int32)
+    return n_ev, offsets, kernels.make_strata(st["n_strat"], st["n_cubes"], n_ev, offsets)
+
+
+@pytest.mark.parametrize("d,n", [(2, 10000), (4, 100000), (3, 50000)])
+def test_vplus_sample_bitwise(d, n):
+    rng = np.random.default_rng(d)
+    st = _setup(d, n, True, rng)
+    grid = random_grid(rng, d)
+    n_ev, offsets, strata = _device_strata(st)
+    total = int(st["n_ev"].sum())
+    step = kernels.make_step(d, total, precision="f64", train=True, seed=5, draw_id=2, xjac=1.0 / st["n_cubes"])
+    x = torch.empty((total, d), dtype=torch.float64, device=DEV)
+    w = torch.empty(total, dtype=torch.float64, device=DEV)
+    idx = torch.empty((total, d), dtype=torch.uint8, device=DEV)
+    seg = torch.empty(total, dtype=torch.int32, device=DEV)
+    kernels.vplus_sample(step, strata, dev(grid), x, w, idx, seg)
+    u = ora.uniform_f64(5, 2, total, d)
+    cubes = ora.hypercubes(st["n_strat"], d)
+    ox, ow, oind, oseg = ora.generate_samples_in_hypercubes(u, st["n_strat"], st["n_ev"], cubes, grid)
+    ox, ow = ora.apply_xjac_and_limits(ox, ow, 1.0 / st["n_cubes"])
+    np.testing.assert_array_equal(seg.cpu().numpy(), oseg)
+    np.testing.assert_array_equal(idx.cpu().numpy(), oind)
+    np.testing.assert_array_equal(x.cpu().numpy(), ox)
+    np.testing.assert_array_equal(w.cpu().numpy(), ow)
+
+
+@pytest.mark.parametrize("adaptive", [True, False])
+@pytest.mark.parametrize("d,n", [(2, 20000), (4, 200000), (5, 100000)])
+def test_vplus_fused_iteration_matches_oracle(d, n, adaptive):
+    rng = np.random.default_rng(10 * d + adaptive)
+    st = _setup(d, n, adaptive, rng)
+    grid = random_grid(rng, d)
+    n_ev, offsets, strata = _device_strata(st)
+    total = int(st["n_ev"].sum())
+    ress = torch.zeros(st["n_cubes"], dtype=torch.float64, device=DEV)
+    ress2 = torch.zeros_like(ress)
+    ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, True))
+    g = dev(grid)
+    step = kernels.make_step(d, total + 1000, precision="f64", train=True, seed=17, draw_id=3,
+                             xjac=1.0 / st["n_cubes"])
+    ng = kernels.vplus_fused(step, strata, MIX.to_struct(d), g, ress, ress2, ws)
+    arr_var = torch.empty(st["n_cubes"], dtype=torch.float64, device=DEV)
+    hist = torch.empty((d, 50), dtype=torch.float64, device=DEV)
+    out = kernels.vplus_finalize(d, strata, n_ev, offsets, ress, ress2, st["adaptive"], st["min_neval"],
+                                 st["init_calls"], ora.BETA, ws, ng, True, True, g, arr_var, hist)
+    torch.cuda.synchronize()
+    u = ora.uniform_f64(17, 3, total, d)
+    res, sig, new_div, new_n_ev, o_var, o_arr = ora.vplus_iteration(grid, st, u, mix_np)
+    o = out.cpu().numpy()
+    assert rel(o[0], res) < 1e-12
+    assert rel(o[1], sig) < 1e-10
+    assert rel(hist.cpu().numpy(), o_arr) < 1e-12
+    np.testing.assert_allclose(g.cpu().numpy(), new_div, rtol=1e-12, atol=1e-15)
+    scale = np.abs(o_var).max()
+    np.testing.assert_allclose(arr_var.cpu().numpy(), o_var, rtol=1e-9, atol=1e-12 * scale)
+    got_n_ev = n_ev.cpu().numpy()
+    diff = np.abs(got_n_ev.astype(np.int64) - new_n_ev.astype(np.int64))
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    off = offsets.cpu().numpy()
+    np.testing.assert_array_equal(off[1:], np.cumsum(got_n_ev))
+    assert int(o[2]) == int(got_n_ev.sum())
+    # per-cube accumulators are zeroed for the next iteration
+    assert float(ress.abs().max()) == 0.0 and float(ress2.abs().max()) == 0.0
+
+
+def test_vplus_digest_path_matches_fused():
+    d, n = 4, 100000
+    rng = np.random.default_rng(3)
+    st = _setup(d, n, True, rng)
+    grid = random_grid(rng, d)
+    total = int(st["n_ev"].sum())
+    outs = []
+    for path in ("fused", "unfused"):
+        n_ev, offsets, strata = _device_strata(st)
+        g = dev(grid)
+        ress = torch.zeros(st["n_cubes"], dtype=torch.float64, device=DEV)
+        ress2 = torch.zeros_like(ress)
+        ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, True))
+        step = kernels.make_step(d, total, precision="f64", train=True, seed=8, xjac=1.0 / st["n_cubes"])
+        if path == "fused":
+            ng = kernels.vplus_fused(step, strata, MIX.to_struct(d), g, ress, ress2, ws)
+        else:
+            x = torch.empty((total, d), dtype=torch.float64, device=DEV)
+            w = torch.empty(total, dtype=torch.float64, device=DEV)
+            idx = torch.empty((total, d), dtype=torch.uint8, device=DEV)
+            seg = torch.empty(total, dtype=torch.int32, device=DEV)
+            kernels.vplus_sample(step, strata, g, x, w, idx, seg)
+            f = MIX(x)
+            ng = kernels.vplus_digest(step, strata, f, w, idx, seg, ress, ress2, ws)
+        out = kernels.vplus_finalize(d, strata, n_ev, offsets, ress, ress2, True, st["min_neval"],
+                                     st["init_calls"], ora.BETA, ws, ng, True, True, g)
+        outs.append((out.cpu().numpy(), g.cpu().numpy(), n_ev.cpu().numpy()))
+    (o1, g1, e1), (o2, g2, e2) = outs
+    assert rel(o1[0], o2[0]) < 1e-12
+    assert rel(o1[1], o2[1]) < 1e-9
+    np.testing.assert_allclose(g1, g2, rtol=1e-12, atol=1e-15)
+    assert np.abs(e1.astype(int) - e2.astype(int)).max() <= 1
+
+
+def test_vplus_mixed_tracks_f64():
+    d, n = 4, 1_000_000
+    st = ora.vplus_setup(d, n, True)
+    grid = ora.initial_grid(d)
+    res = {}
+    for prec in ("f64", "mixed"):
+        n_ev, offsets, strata = _device_strata(st)
+        g = dev(grid)
+        ress = torch.zeros(st["n_cubes"], dtype=torch.float64, device=DEV)
+        ress2 = torch.zeros_like(ress)
+        ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, True))
+        total = int(st["n_ev"].sum())
+        step = kernels.make_step(d, total, precision=prec, train=True, seed=23, xjac=1.0 / st["n_cubes"])
+        ng = kernels.vplus_fused(step, strata, MIX.to_struct(d), g, ress, ress2, ws)
+        out = kernels.vplus_finalize(d, strata, n_ev, offsets, ress, ress2, True, st["min_neval"],
+                                     st["init_calls"], ora.BETA, ws, ng, True, True, g)
+        res[prec] = out.cpu().numpy()
+    assert abs(res["mixed"][0] - res["f64"][0]) < 0.05 * res["f64"][1]
+    assert abs(res["mixed"][1] - res["f64"][1]) < 0.01 * res["f64"][1]

@@ -16,11 +16,8 @@ FUSED_MAX_DIM = 16
 MAX_COMP = 4
 MAX_OUT = 16
 MAX_BLOCKS = 2048
-GROUP = 64
-MAX_GROUPS = MAX_BLOCKS // GROUP
+ROWS = 32
 DIGEST_DIMS = 16
-MAX_SLICES = 64
-TICKETS = MAX_GROUPS * MAX_SLICES
 
 F64 = 0
 MIXED = 1
@@ -60,7 +57,7 @@ class Step(ctypes.Structure):
         ("n_dim", _i32),
         ("precision", _i32),
         ("train", _i32),
-        ("accumulate", _i32),
+        ("reserved", _i32),
         ("n_events", _i64),
         ("event_offset", _i64),
         ("seed", _u64),
@@ -76,9 +73,7 @@ class Step(ctypes.Structure):
 
 class Partials(ctypes.Structure):
     _fields_ = [
-        ("block_rows", _vp),
-        ("group_rows", _vp),
-        ("tickets", _vp),
+        ("rows", _vp),
         ("stride", _i64),
     ]
 
@@ -114,7 +109,7 @@ SIGNATURES = {
     "mv_struct_size": (_i64, [_i32]),
     "mv_partial_stride": (_i64, [_i32, _i32, _i32]),
     "mv_producer_blocks": (_i32, [_i64]),
-    "mv_producer_groups": (_i32, [_i64]),
+    "mv_producer_rows": (_i32, [_i64]),
     "mv_vegas_fused": (ctypes.c_int, [_P(Step), _P(Integrand), _vp, _P(Partials), _vp]),
     "mv_vegas_sample": (ctypes.c_int, [_P(Step), _vp, _vp, _vp, _vp, _vp]),
     "mv_vegas_digest": (ctypes.c_int, [_P(Step), _vp, _i64, _i32, _i32, _vp, _i64, _vp,

@@ -54,17 +54,25 @@ struct FzK {
   double n_events;
 };
 
+// Sum column c of the group rows in order and zero it for the next producer.
+__device__ __forceinline__ double take_column(double* __restrict__ P, int64_t stride, int nrows,
+                                              int64_t c) {
+  double a = P[c];
+  P[c] = 0.0;
+  for (int r = 1; r < nrows; ++r) {
+    a += P[(int64_t)r * stride + c];
+    P[(int64_t)r * stride + c] = 0.0;
+  }
+  return a;
+}
+
 // Sums and sigma from the group rows (vflow.py:432-442): thread v < 2 n_out
-// adds column v over the rows in order.
-__device__ inline void finalize_sums_block(const FzK& z, const double* __restrict__ P, int64_t stride,
+// takes column v.
+__device__ inline void finalize_sums_block(const FzK& z, double* __restrict__ P, int64_t stride,
                                            int nrows, double* __restrict__ out) {
   __shared__ double red[2 * MV_MAX_OUT];
   const int t = threadIdx.x;
-  if (t < 2 * z.n_out) {
-    double a = P[t];
-    for (int r = 1; r < nrows; ++r) a += P[(int64_t)r * stride + t];
-    red[t] = a;
-  }
+  if (t < 2 * z.n_out) red[t] = take_column(P, stride, nrows, t);
   __syncthreads();
   if (t < z.n_out) {
     const double N = z.n_events;
@@ -80,17 +88,16 @@ __device__ inline void finalize_sums_block(const FzK& z, const double* __restric
   }
 }
 
-// Histogram of dimension j from the group rows (fixed order), exported to
-// hist_out and/or used to refine grid row j (vflow.py:349-362).
-__device__ inline void hist_refine_block(const FzK& z, const double* __restrict__ P, int64_t stride,
+// Histogram of dimension j from the group rows (fixed order, columns zeroed
+// afterwards), exported to hist_out and/or used to refine grid row j
+// (vflow.py:349-362).
+__device__ inline void hist_refine_block(const FzK& z, double* __restrict__ P, int64_t stride,
                                          int nrows, int j, double* __restrict__ grid,
                                          double* __restrict__ hist_out) {
   __shared__ double hb[kBins];
   const int t = threadIdx.x;
   if (t < kBins) {
-    const int64_t c = 2 * z.n_out + j * kBins + t;
-    double a = P[c];
-    for (int r = 1; r < nrows; ++r) a += P[(int64_t)r * stride + c];
+    const double a = take_column(P, stride, nrows, 2 * z.n_out + j * kBins + t);
     hb[t] = a;
     if (hist_out) hist_out[j * kBins + t] = a;
   }

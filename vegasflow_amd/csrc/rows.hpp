@@ -1,12 +1,15 @@
-// rows.hpp - launch parameters shared by the producers and the two-level
-// fixed-order reduction of their partial rows.
+// rows.hpp - launch parameters shared by the producers, and how their
+// partial sums reach the finalize kernels.
 //
-// Every workgroup of a producer writes one block row. The last workgroup to
-// finish within each group of kGroup consecutive workgroups (it learns so
-// from a ticket counter it then resets to zero) adds the group's block rows,
-// in increasing workgroup order, into one group row. The finalize kernels
-// then reduce at most MV_MAX_GROUPS group rows. No spin waits: a workgroup
-// that is not last simply exits.
+// Every workgroup of a producer reduces its events to one row
+// [sum_k, sum_k^2, histogram] in registers and LDS, then adds that row into
+// group row (blockIdx.x mod MV_ROWS) with FP64 global atomics at agent scope.
+// Atomics are coherent across the XCDs without any L2 write-back or
+// invalidate, which a fence-based "last workgroup reduces" scheme would pay
+// once per workgroup. The finalize kernels read the MV_ROWS group rows in a
+// fixed order and zero them for the next producer, so the rows are zero
+// whenever no iteration is in flight. The order in which workgroups land in a
+// group row is not fixed: the sums are reproducible to rounding, not bitwise.
 #pragma once
 
 #include "abi_util.hpp"
@@ -14,7 +17,7 @@
 
 namespace mv {
 
-constexpr int kGroup = MV_GROUP;
+constexpr int kRows = MV_ROWS;
 
 struct StepK {
   int n_dim;
@@ -48,11 +51,8 @@ inline StepK make_stepk(const mv_step& s) {
 }
 
 struct RowsK {
-  double* blk;
-  double* grp;
-  int32_t* tickets;
+  double* rows;
   int64_t stride;
-  int accumulate;
 };
 
 inline int producer_blocks(int64_t n_events) {
@@ -62,46 +62,57 @@ inline int producer_blocks(int64_t n_events) {
   return (int)b;
 }
 
-inline int groups_of(int blocks) { return (blocks + kGroup - 1) / kGroup; }
+inline int rows_used(int blocks) { return blocks < kRows ? blocks : kRows; }
 
-inline int make_rows(const mv_partials* p, int64_t need_stride, int accumulate, const char* what,
-                     RowsK* out) {
-  if (!p || !p->block_rows || !p->group_rows || !p->tickets)
-    return fail(MV_ERR_ARG, "%s: incomplete partials workspace", what);
+inline int make_rows(const mv_partials* p, int64_t need_stride, const char* what, RowsK* out) {
+  if (!p || !p->rows) return fail(MV_ERR_ARG, "%s: partials workspace missing", what);
   if (p->stride < need_stride)
     return fail(MV_ERR_WORKSPACE, "%s: row stride %lld < %lld", what, (long long)p->stride,
                 (long long)need_stride);
-  out->blk = p->block_rows;
-  out->grp = p->group_rows;
-  out->tickets = p->tickets;
+  out->rows = p->rows;
   out->stride = p->stride;
-  out->accumulate = accumulate != 0;
   return MV_OK;
 }
 
-// Fold columns [c0, c1) of this workgroup's group; ``slot`` selects an
-// independent ticket set (grid.y slices of the digest kernel).
-__device__ inline void group_fold_cols(const RowsK& R, int c0, int c1, int slot) {
-  __shared__ int last;
-  __threadfence();  // publish this workgroup's block row device-wide
-  __syncthreads();
-  const int g = blockIdx.x / kGroup;
-  const int r0 = g * kGroup;
-  const int gs = min(kGroup, (int)gridDim.x - r0);
-  int32_t* tk = R.tickets + (int64_t)slot * MV_MAX_GROUPS + g;
-  if (threadIdx.x == 0) last = (atomicAdd(tk, 1) == gs - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();  // acquire the other block rows of the group
-  for (int c = c0 + (int)threadIdx.x; c < c1; c += (int)blockDim.x) {
-    double s = R.blk[(int64_t)r0 * R.stride + c];
-    for (int r = 1; r < gs; ++r) s += R.blk[(int64_t)(r0 + r) * R.stride + c];
-    double* o = R.grp + (int64_t)g * R.stride + c;
-    *o = R.accumulate ? *o + s : s;
-  }
-  if (threadIdx.x == 0) *tk = 0;
+// this workgroup's group row
+__device__ __forceinline__ double* group_row(const RowsK& R) {
+  return R.rows + (int64_t)(blockIdx.x % kRows) * R.stride;
 }
 
-__device__ inline void group_fold(const RowsK& R, int ncols) { group_fold_cols(R, 0, ncols, 0); }
+__device__ __forceinline__ void row_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// Workgroup end of the fused producers: reduce the per-thread sums over the
+// workgroup (fixed order) and the kWaves private LDS histograms, and add the
+// resulting row [sum, sum2, hist(hist_len)] into this workgroup's group row.
+// SUMS = false leaves columns 0 and 1 alone (VEGAS+ sums per hypercube).
+template <bool TRAIN, bool SUMS = true>
+__device__ __forceinline__ void finish_block(double acc, double acc2, const double* hist,
+                                             int hist_len, const RowsK& R) {
+  __shared__ double red[2 * kWaves];
+  double* row = group_row(R);
+  if (SUMS) {
+    acc = wave_sum(acc);
+    acc2 = wave_sum(acc2);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      red[wave] = acc;
+      red[kWaves + wave] = acc2;
+    }
+  }
+  __syncthreads();
+  if (SUMS && threadIdx.x < 2) {
+    const double* r = red + threadIdx.x * kWaves;
+    double a = r[0];
+    for (int k = 1; k < kWaves; ++k) a += r[k];
+    row_add(&row[threadIdx.x], a);
+  }
+  if (TRAIN) {
+    for (int i = threadIdx.x; i < hist_len; i += kBlock) {
+      double h = hist[i];
+      for (int k = 1; k < kWaves; ++k) h += hist[k * hist_len + i];
+      if (h != 0.0) row_add(&row[2 + i], h);
+    }
+  }
+}
 
 }  // namespace mv

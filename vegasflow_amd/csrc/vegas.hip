@@ -3,10 +3,10 @@
 // Kernels
 //   vegas_fused_f64 / vegas_fused_mixed   draws + grid map + built-in
 //       integrand + sums + per-wave LDS bin histograms; no per-event HBM
-//       traffic. Each workgroup writes one block row of partial sums.
+//       traffic. Each workgroup adds one row of partial sums into a group row.
 //   vegas_sample_{f64,mixed} / vegas_digest   the unfused path around a
 //       torch-callable integrand (x, w, idx through HBM).
-//   finalize_kernel   fixed-order reduction of group rows, sigma, and the
+//   finalize_kernel   fixed-order reduction of group rows (zeroed after), sigma, and the
 //       per-dimension refinement (one workgroup per dimension).
 //   plain_*   PlainFlow variants.
 #include <algorithm>
@@ -25,40 +25,6 @@ std::string& last_error_slot() {
 }
 
 __device__ __forceinline__ int clamp_bin(int b) { return b < 0 ? 0 : (b > kBins - 1 ? kBins - 1 : b); }
-
-// Block row [sum, sum2, hist(d*50)] (n_out = 1) from per-wave registers and
-// per-wave LDS histograms, then the group fold.
-template <bool TRAIN>
-__device__ __forceinline__ void finish_block(double acc, double acc2, const double* hist,
-                                             int hist_len, const RowsK& R) {
-  __shared__ double red[2 * kWaves];
-  acc = wave_sum(acc);
-  acc2 = wave_sum(acc2);
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[wave] = acc;
-    red[kWaves + wave] = acc2;
-  }
-  __syncthreads();
-  double* row = R.blk + (int64_t)blockIdx.x * R.stride;
-  if (threadIdx.x == 0) {
-    double a = red[0], b = red[kWaves];
-    for (int k = 1; k < kWaves; ++k) {
-      a += red[k];
-      b += red[kWaves + k];
-    }
-    row[0] = a;
-    row[1] = b;
-  }
-  if (TRAIN) {
-    for (int i = threadIdx.x; i < hist_len; i += kBlock) {
-      double h = hist[i];
-      for (int k = 1; k < kWaves; ++k) h += hist[k * hist_len + i];
-      row[2 + i] = h;
-    }
-  }
-  group_fold(R, 2 + (TRAIN ? hist_len : 0));
-}
 
 // ---------------------------------------------------------------------------
 // fused VegasFlow, FP64 (vflow.py:87-126 + 33-83 + 389-430,
@@ -163,8 +129,8 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_mixed(StepK s, mv_integran
   extern __shared__ double smem[];
   const int d = s.n_dim;
   const int nb = d * kBins;
-  float2* tab = reinterpret_cast<float2*>(smem);          // nb float2
-  float* tw = reinterpret_cast<float*>(smem + nb);         // nb float (nb/2 doubles)
+  float2* tab = reinterpret_cast<float2*>(smem);   // nb float2
+  float* tw = reinterpret_cast<float*>(smem + nb);  // nb float (nb/2 doubles)
   double* hist = smem + nb + (nb + 1) / 2;
   const bool limits = s.xmin != nullptr;
   for (int i = threadIdx.x; i < nb; i += kBlock) {
@@ -477,28 +443,26 @@ __global__ __launch_bounds__(kBlock) void vegas_digest(StepK s, const FT* __rest
     }
   }
   __syncthreads();
-  double* row = R.blk + (int64_t)blockIdx.x * R.stride;
+  // every y-slice adds its own histogram columns; slice 0 owns the sums
+  double* row = group_row(R);
   if (sums && threadIdx.x < 2 * no) {
     const int v = threadIdx.x;  // 2k: sum of output k, 2k+1: its sum of squares
     double a = red[v * kWaves];
     for (int q = 1; q < kWaves; ++q) a += red[v * kWaves + q];
-    row[(v & 1) ? no + (v >> 1) : (v >> 1)] = a;
+    row_add(&row[(v & 1) ? no + (v >> 1) : (v >> 1)], a);
   }
   if (TRAIN) {
     for (int i = threadIdx.x; i < nb; i += kBlock) {
       double h = hist[i];
       for (int q = 1; q < kWaves; ++q) h += hist[q * nb + i];
-      row[2 * no + j0 * kBins + i] = h;
+      if (h != 0.0) row_add(&row[2 * no + j0 * kBins + i], h);
     }
   }
-  // every y-slice folds its own columns; slice 0 owns the sums
-  group_fold_cols(R, sums ? 0 : 2 * no + j0 * kBins, 2 * no + j0 * kBins + (TRAIN ? nb : 0),
-                  blockIdx.y);
 }
 // ---------------------------------------------------------------------------
 // finalize: block 0 sums and sigma, block 1 + j dimension j (refine.hpp)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void finalize_kernel(FzK z, const double* __restrict__ P,
+__global__ __launch_bounds__(kBlock) void finalize_kernel(FzK z, double* __restrict__ P,
                                                           int64_t stride, int nrows,
                                                           double* __restrict__ grid,
                                                           double* __restrict__ out,
@@ -581,8 +545,8 @@ extern "C" int64_t mv_partial_stride(int32_t n_dim, int32_t n_out, int32_t train
 
 extern "C" int32_t mv_producer_blocks(int64_t n_events) { return producer_blocks(n_events); }
 
-extern "C" int32_t mv_producer_groups(int64_t n_events) {
-  return groups_of(producer_blocks(n_events));
+extern "C" int32_t mv_producer_rows(int64_t n_events) {
+  return rows_used(producer_blocks(n_events));
 }
 
 namespace {
@@ -631,7 +595,7 @@ extern "C" int mv_vegas_fused(const mv_step* step, const mv_integrand* fn, const
   if ((rc = validate_fn(fn, "mv_vegas_fused"))) return rc;
   if (!grid) return fail(MV_ERR_ARG, "mv_vegas_fused: grid is required");
   RowsK R;
-  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train), step->accumulate,
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train),
                       "mv_vegas_fused", &R)))
     return rc;
   const int blocks = producer_blocks(step->n_events);
@@ -646,7 +610,7 @@ extern "C" int mv_vegas_fused(const mv_step* step, const mv_integrand* fn, const
     case MV_FN_BWPOWER: launch_vegas_fused<MV_FN_BWPOWER>(k, *fn, mixed, train, grid, R, blocks, st); break;
   }
   if ((rc = check_launch("mv_vegas_fused"))) return rc;
-  return groups_of(blocks);
+  return rows_used(blocks);
 }
 
 extern "C" int mv_plain_fused(const mv_step* step, const mv_integrand* fn, const mv_partials* parts,
@@ -659,7 +623,7 @@ extern "C" int mv_plain_fused(const mv_step* step, const mv_integrand* fn, const
     return fail(MV_ERR_UNSUPPORTED, "mv_plain_fused: n_dim %d exceeds %d for this family",
                 step->n_dim, MV_FUSED_MAX_DIM);
   RowsK R;
-  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, 0), step->accumulate,
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, 0),
                       "mv_plain_fused", &R)))
     return rc;
   const int blocks = producer_blocks(step->n_events);
@@ -674,7 +638,7 @@ extern "C" int mv_plain_fused(const mv_step* step, const mv_integrand* fn, const
     case MV_FN_BWPOWER: launch_plain_fused<MV_FN_BWPOWER>(k, *fn, mixed, R, blocks, st); break;
   }
   if ((rc = check_launch("mv_plain_fused"))) return rc;
-  return groups_of(blocks);
+  return rows_used(blocks);
 }
 
 extern "C" int mv_vegas_sample(const mv_step* step, const double* grid, void* x, void* w,
@@ -724,16 +688,14 @@ extern "C" int mv_vegas_digest(const mv_step* step, const void* f, int64_t f_str
   if (f_stride < n_out) return fail(MV_ERR_ARG, "mv_vegas_digest: f_stride < n_out");
   if (w_stride < 0) return fail(MV_ERR_ARG, "mv_vegas_digest: negative w_stride");
   RowsK R;
-  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, n_out, step->train), step->accumulate,
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, n_out, step->train),
                       "mv_vegas_digest", &R)))
     return rc;
   const int blocks = producer_blocks(step->n_events);
   const StepK k = make_stepk(*step);
   hipStream_t st = as_stream(stream);
   const int slices = step->train ? (step->n_dim + kDigestDims - 1) / kDigestDims : 1;
-  if (slices > MV_MAX_SLICES)
-    return fail(MV_ERR_UNSUPPORTED, "mv_vegas_digest: n_dim %d exceeds %d", step->n_dim,
-                MV_MAX_SLICES * kDigestDims);
+  if (slices > 65535) return fail(MV_ERR_UNSUPPORTED, "mv_vegas_digest: n_dim %d too large", step->n_dim);
   const dim3 g(blocks, slices);
   const bool vec = n_out > 1, train = step->train != 0;
 #define MV_DIGEST(FT, V, TR)                                                                   \
@@ -749,10 +711,10 @@ extern "C" int mv_vegas_digest(const mv_step* step, const void* f, int64_t f_str
   }
 #undef MV_DIGEST
   if ((rc = check_launch("mv_vegas_digest"))) return rc;
-  return groups_of(blocks);
+  return rows_used(blocks);
 }
 
-extern "C" int mv_vegas_finalize(const mv_finalize* fz, const double* rows, int64_t row_stride,
+extern "C" int mv_vegas_finalize(const mv_finalize* fz, double* rows, int64_t row_stride,
                                  int32_t n_rows, double* grid, double* out, double* hist_out,
                                  void* stream) {
   if (!fz) return fail(MV_ERR_ARG, "mv_vegas_finalize: fz is NULL");
@@ -765,7 +727,7 @@ extern "C" int mv_vegas_finalize(const mv_finalize* fz, const double* rows, int6
     return fail(MV_ERR_ARG, "mv_vegas_finalize: refine needs train rows and a grid");
   if (hist_out && !fz->train) return fail(MV_ERR_ARG, "mv_vegas_finalize: no histogram without train");
   FzK z{fz->n_dim, fz->n_out, fz->train, fz->refine, fz->raw, fz->n_events};
-  const int blocks = 1 + ((fz->train && (fz->refine || hist_out)) ? fz->n_dim : 0);
+  const int blocks = 1 + (fz->train ? fz->n_dim : 0);  // histogram columns are always consumed
   hipLaunchKernelGGL(finalize_kernel, dim3(blocks), dim3(kBlock), 0, as_stream(stream), z, rows,
                      row_stride, n_rows, grid, out, hist_out);
   return check_launch("mv_vegas_finalize");

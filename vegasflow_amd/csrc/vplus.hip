@@ -89,37 +89,6 @@ __device__ __forceinline__ void segment_add(int cube, double v, double v2, doubl
 
 __device__ __forceinline__ int clamp_bin(int b) { return b < 0 ? 0 : (b > kBins - 1 ? kBins - 1 : b); }
 
-template <bool TRAIN>
-__device__ __forceinline__ void finish_block(double acc, double acc2, const double* hist,
-                                             int hist_len, const RowsK& R) {
-  __shared__ double red[2 * kWaves];
-  acc = wave_sum(acc);
-  acc2 = wave_sum(acc2);
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[wave] = acc;
-    red[kWaves + wave] = acc2;
-  }
-  __syncthreads();
-  double* row = R.blk + (int64_t)blockIdx.x * R.stride;
-  if (threadIdx.x == 0) {
-    double a = red[0], b = red[kWaves];
-    for (int k = 1; k < kWaves; ++k) {
-      a += red[k];
-      b += red[kWaves + k];
-    }
-    row[0] = a;
-    row[1] = b;
-  }
-  if (TRAIN) {
-    for (int i = threadIdx.x; i < hist_len; i += kBlock) {
-      double h = hist[i];
-      for (int k = 1; k < kWaves; ++k) h += hist[k * hist_len + i];
-      row[2 + i] = h;
-    }
-  }
-  group_fold(R, 2 + (TRAIN ? hist_len : 0));
-}
 
 // ---------------------------------------------------------------------------
 // fused VEGAS+ (vflowplus.py:37-80 + 187-220, monte_carlo.py:249-275)
@@ -237,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_in
     }
     segment_add(cube, tmp, tmp2, ress, ress2);
   }
-  finish_block<TRAIN>(acc, acc2, hist, nb, R);
+  finish_block<TRAIN, false>(acc, acc2, hist, nb, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -358,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void vplus_digest(StepK s, StrataK st, cons
     }
     segment_add(cube, tmp, tmp2, ress, ress2);
   }
-  finish_block<TRAIN>(acc, acc2, hist, nb, R);
+  finish_block<TRAIN, false>(acc, acc2, hist, nb, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -444,7 +413,7 @@ __global__ __launch_bounds__(kCubeThreads) void vplus_cubes_kernel(
   }
 }
 
-__global__ __launch_bounds__(kBlock) void vplus_hist_kernel(FzK z, const double* __restrict__ P,
+__global__ __launch_bounds__(kBlock) void vplus_hist_kernel(FzK z, double* __restrict__ P,
                                                             int64_t stride, int nrows,
                                                             double* __restrict__ grid,
                                                             double* __restrict__ hist_out) {
@@ -495,7 +464,7 @@ extern "C" int mv_vplus_fused(const mv_step* step, const mv_strata* strata, cons
   if (!grid || !ress || !ress2) return fail(MV_ERR_ARG, "mv_vplus_fused: grid, ress, ress2 required");
   if (step->n_events >= (int64_t)1 << 31) return fail(MV_ERR_UNSUPPORTED, "mv_vplus_fused: too many events");
   RowsK R;
-  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train), step->accumulate,
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train),
                       "mv_vplus_fused", &R)))
     return rc;
   const int blocks = producer_blocks(step->n_events);
@@ -511,7 +480,7 @@ extern "C" int mv_vplus_fused(const mv_step* step, const mv_strata* strata, cons
     case MV_FN_BWPOWER: launch_vplus_fused<MV_FN_BWPOWER>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
   }
   if ((rc = check_launch("mv_vplus_fused"))) return rc;
-  return groups_of(blocks);
+  return rows_used(blocks);
 }
 
 extern "C" int mv_vplus_sample(const mv_step* step, const mv_strata* strata, const double* grid,
@@ -543,7 +512,7 @@ extern "C" int mv_vplus_digest(const mv_step* step, const mv_strata* strata, con
   if (step->train && step->n_dim > 40)
     return fail(MV_ERR_UNSUPPORTED, "mv_vplus_digest: n_dim > 40 with training");
   RowsK R;
-  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train), step->accumulate,
+  if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train),
                       "mv_vplus_digest", &R)))
     return rc;
   const int blocks = producer_blocks(step->n_events);
@@ -558,12 +527,12 @@ extern "C" int mv_vplus_digest(const mv_step* step, const mv_strata* strata, con
   else { if (step->train) MV_VPD(double, true); else MV_VPD(double, false); }
 #undef MV_VPD
   if ((rc = check_launch("mv_vplus_digest"))) return rc;
-  return groups_of(blocks);
+  return rows_used(blocks);
 }
 
 extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t* n_ev, int32_t* offsets,
                                  double* ress, double* ress2, int32_t adaptive, int32_t min_neval,
-                                 double init_calls, double beta, const double* rows, int64_t row_stride,
+                                 double init_calls, double beta, double* rows, int64_t row_stride,
                                  int32_t n_rows, int32_t train, int32_t refine, double* grid,
                                  double* out, double* arr_var_out, double* hist_out, void* stream) {
   int rc = validate_strata(strata, n_dim, "mv_vplus_finalize");
@@ -575,7 +544,7 @@ extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t
   hipLaunchKernelGGL(vplus_cubes_kernel, dim3(1), dim3(kCubeThreads), 0, s, strata->n_cubes, n_ev,
                      offsets, ress, ress2, adaptive, min_neval, init_calls, beta, out, arr_var_out);
   if ((rc = check_launch("mv_vplus_finalize"))) return rc;
-  if (train && (refine || hist_out)) {
+  if (train) {  // histogram columns are always consumed
     if (!rows || n_rows < 1) return fail(MV_ERR_ARG, "mv_vplus_finalize: histogram rows required");
     if (row_stride < mv_partial_stride(n_dim, 1, 1))
       return fail(MV_ERR_ARG, "mv_vplus_finalize: row_stride too small");

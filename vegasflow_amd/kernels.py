@@ -55,19 +55,15 @@ def float_dtype(precision):
 
 
 class Workspace:
-    """Device buffers for the partial rows of producer launches (caller-owned
-    memory of the ABI). Tickets start at zero and every launch leaves them
-    at zero."""
+    """Group rows of partial sums (caller-owned memory of the ABI): zero
+    when allocated; producers add into them and the finalize calls consume
+    and re-zero them."""
 
     def __init__(self, device, stride):
         self.device = device
         self.stride = int(stride)
-        self.block_rows = torch.empty(_abi.MAX_BLOCKS * self.stride, dtype=torch.float64, device=device)
-        self.group_rows = torch.zeros(_abi.MAX_GROUPS * self.stride, dtype=torch.float64, device=device)
-        self.tickets = torch.zeros(_abi.TICKETS, dtype=torch.int32, device=device)
-        self._struct = _abi.Partials(
-            self.block_rows.data_ptr(), self.group_rows.data_ptr(), self.tickets.data_ptr(), self.stride
-        )
+        self.rows = torch.zeros(_abi.ROWS * self.stride, dtype=torch.float64, device=device)
+        self._struct = _abi.Partials(self.rows.data_ptr(), self.stride)
 
     def partials(self):
         return ctypes.byref(self._struct)
@@ -91,11 +87,11 @@ def partial_stride(n_dim, n_out=1, train=True):
     return 2 * n_out + (n_dim * BINS if train else 0)
 
 
-def producer_groups(n_events):
-    return _abi.load().mv_producer_groups(int(n_events))
+def producer_rows(n_events):
+    return _abi.load().mv_producer_rows(int(n_events))
 
 
-def make_step(n_dim, n_events, *, precision="f64", train=True, accumulate=False, event_offset=0,
+def make_step(n_dim, n_events, *, precision="f64", train=True, event_offset=0,
               seed=0, draw_id=0, xjac=1.0, xmin=None, xdelta=None, xdeltajac=1.0, inject=None):
     _need_device(xmin, "xmin", torch.float64)
     _need_device(xdelta, "xdelta", torch.float64)
@@ -110,7 +106,6 @@ def make_step(n_dim, n_events, *, precision="f64", train=True, accumulate=False,
     s.n_dim = int(n_dim)
     s.precision = precision_code(precision)
     s.train = int(bool(train))
-    s.accumulate = int(bool(accumulate))
     s.n_events = int(n_events)
     s.event_offset = int(event_offset)
     s.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
@@ -127,7 +122,7 @@ def make_step(n_dim, n_events, *, precision="f64", train=True, accumulate=False,
 # VegasFlow / PlainFlow
 # ---------------------------------------------------------------------------
 def vegas_fused(step, fn_struct, grid, ws):
-    """mv_vegas_fused; returns the number of group rows in ``ws``."""
+    """mv_vegas_fused; returns the number of group rows of ``ws`` touched."""
     _need_device(grid, "grid", torch.float64)
     return _abi.call("mv_vegas_fused", ctypes.byref(step), ctypes.byref(fn_struct), _ptr(grid),
                      ws.partials(), _stream())
@@ -191,7 +186,7 @@ def vegas_finalize(n_dim, n_out, train, refine, n_events, ws, n_rows, grid=None,
     fz.refine = int(bool(refine))
     fz.raw = int(bool(raw))
     fz.n_events = float(n_events)
-    _abi.call("mv_vegas_finalize", ctypes.byref(fz), _ptr(ws.group_rows), ws.stride, int(n_rows),
+    _abi.call("mv_vegas_finalize", ctypes.byref(fz), _ptr(ws.rows), ws.stride, int(n_rows),
               _ptr(grid), _ptr(out), _ptr(hist_out), _stream())
     return out
 
@@ -273,6 +268,6 @@ def vplus_finalize(n_dim, strata, n_ev, offsets, ress, ress2, adaptive, min_neva
     out = torch.empty(3, dtype=torch.float64, device=ws.device)
     _abi.call("mv_vplus_finalize", int(n_dim), ctypes.byref(strata), _ptr(n_ev), _ptr(offsets), _ptr(ress),
               _ptr(ress2), int(bool(adaptive)), int(min_neval), float(init_calls), float(beta),
-              _ptr(ws.group_rows), ws.stride, int(n_rows), int(bool(train)), int(bool(refine)),
+              _ptr(ws.rows), ws.stride, int(n_rows), int(bool(train)), int(bool(refine)),
               _ptr(grid), _ptr(out), _ptr(arr_var_out), _ptr(hist_out), _stream())
     return out
