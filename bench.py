@@ -85,6 +85,71 @@ def protocol(vf, dtype, fn, d, n, seed):
     return float(res), float(sig)
 
 
+def bench_vplus(vf, dtype, n_iter=10):
+    """VegasFlowPlus, d=4 two-Gaussian mixture, 1e6 events requested,
+    adaptive (5e5 events in the first iteration), 10 iterations."""
+    import torch
+
+    fn = vf.GaussianMix((0.5, 0.5), (0.05, 0.1), (0.33, 0.67))
+    inst = vf.VegasFlowPlus(4, 1_000_000, adaptive=True, seed=SEED, verbose=False, dtype=dtype)
+    inst.compile(fn)
+    inst.run_integration(1)  # warm-up (code objects, workspace)
+    inst = vf.VegasFlowPlus(4, 1_000_000, adaptive=True, seed=SEED, verbose=False, dtype=dtype)
+    inst.compile(fn)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res, sig = inst.run_integration(n_iter)
+    dt = time.perf_counter() - t0
+    events = sum(inst.events_history)
+    return {"integral": float(res), "sigma": float(sig), "pull_vs_exact": (float(res) - fn.exact(4)) / float(sig),
+            "events_per_s": events / dt, "ms_per_iteration": 1e3 * dt / n_iter, "events": events,
+            "iterations": n_iter, "events_first_iteration": inst.events_history[0],
+            "events_last_iteration": inst.events_history[-1]}
+
+
+def bench_batched(torch, dtype, n_iter=8):
+    """256 Gaussian integrals (d ~ U{2..12}, events ~ logU[1e4, 1e6],
+    a ~ U[0.05, 0.3], centres ~ U[0.2, 0.8]^d, seed 23), 8 iterations, each
+    round a replay of one captured HIP graph."""
+    import numpy as np
+    from vegasflow_amd.batch import BatchVegas, random_batch
+
+    specs = random_batch(256, seed=23)
+    b = BatchVegas(specs, dtype=dtype)
+    b.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(n_iter)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res, err = b.results()
+    exact = np.array([fn.exact(d) for fn, d, _, _ in specs])
+    pulls = (res - exact) / err
+    return {"events_per_s": b.events_per_round() * n_iter / dt, "ms_per_round": 1e3 * dt / n_iter,
+            "events_per_round": b.events_per_round(), "items": len(specs),
+            "events_per_tile": b.events_per_tile, "workgroups": b.blocks,
+            "frac_within_3sigma": float(np.mean(np.abs(pulls) < 3)),
+            "median_abs_pull": float(np.median(np.abs(pulls)))}
+
+
+def bench_serving(seconds, rate=200.0):
+    """Poisson stream at ``rate`` req/s for ``seconds``: two-Gaussian mixture
+    (d=4) or Breit-Wigner x power (d=6); target 1e-3 (1e6 events/iteration)
+    or 1e-4 (2e7 events/iteration); at most 20 iterations."""
+    from vegasflow_amd.serve import Server, latency_report, poisson_requests
+
+    reqs = poisson_requests(rate=rate, duration=seconds, seed=5)
+    server = Server(memory_budget=256 << 20)
+    t0 = time.perf_counter()
+    done = server.run_stream(reqs)
+    dt = time.perf_counter() - t0
+    rep = latency_report(done)
+    rep.update({"rate_req_per_s": rate, "duration_s": seconds, "wall_s": dt,
+                "events_per_s": server.events_done / dt, "ticks": server.ticks,
+                "slots": server.capacity})
+    return rep
+
+
 def cpu_baseline(seconds_budget=20.0):
     """The NumPy FP64 oracle (a restatement of the reference algorithm,
     Philox draws included) timed on the host on one core-bound iteration."""
@@ -118,6 +183,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-protocol", action="store_true", help="skip the 5+5 iteration error check")
+    ap.add_argument("--no-extra", action="store_true", help="skip the VEGAS+, batched and serving runs")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the Poisson stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1",
@@ -193,6 +260,12 @@ def main():
         },
     }
     line["roofline"]["f64_kernel"]["frac"] = line["roofline"]["f64_kernel"]["achieved"] / PEAK["f64"]
+    if not args.no_extra:
+        line["vegasflowplus_d4"] = {p: bench_vplus(vf, dt_) for p, dt_ in
+                                    (("mixed", torch.float32), ("f64", torch.float64))}
+        line["batched_256"] = {p: bench_batched(torch, dt_) for p, dt_ in
+                               (("mixed", torch.float32), ("f64", torch.float64))}
+        line["serving"] = bench_serving(args.serve_seconds)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))

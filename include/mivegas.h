@@ -133,7 +133,8 @@ typedef struct {
 int mv_abi_version(void);
 const char* mv_last_error(void);
 /* sizeof of the ABI structs, for binding-layout checks:
- * 0 mv_integrand, 1 mv_step, 2 mv_partials, 3 mv_strata, 4 mv_finalize. */
+ * 0 mv_integrand, 1 mv_step, 2 mv_partials, 3 mv_strata, 4 mv_finalize,
+ * 5 mv_batch_item, 6 mv_batch_tile. */
 int64_t mv_struct_size(int32_t which);
 /* Doubles per partial row: 2 * n_out + (train ? n_dim * MV_BINS : 0). */
 int64_t mv_partial_stride(int32_t n_dim, int32_t n_out, int32_t train);
@@ -225,6 +226,58 @@ int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t* n_ev, int
                       double init_calls, double beta, double* rows, int64_t row_stride,
                       int32_t n_rows, int32_t train, int32_t refine, double* grid, double* out,
                       double* arr_var_out, double* hist_out, void* stream);
+
+/* ---- batched independent integrals ------------------------------------- */
+/* No counterpart in the reference (each VegasFlow instance there runs its
+ * own graph): many VegasFlow integrals of different n_dim (<=
+ * MV_FUSED_MAX_DIM), event count and built-in integrand advance one
+ * iteration each in one persistent launch, with their grids, accumulators
+ * and iteration counters resident on the device. Item i draws exactly what
+ * VegasFlow(n_dim_i, n_events_i, seed = seed_i) would at iteration
+ * draws[i], so its results equal a lone run up to summation order. */
+typedef struct {
+  int32_t n_dim;
+  int32_t train;         /* refine this item's grid in mv_batch_finalize */
+  int64_t n_events;      /* events per iteration (xjac = 1 / n_events) */
+  uint64_t seed;
+  int64_t grid_offset;   /* doubles from ``grids`` to its (n_dim, 51) grid */
+  int64_t row_offset;    /* doubles from ``rows`` to its 2 + 50 n_dim row */
+  mv_integrand fn;
+} mv_batch_item;
+
+/* A tile: ``count`` consecutive events of one item, from event ``first``. */
+typedef struct {
+  int32_t item;
+  int32_t count;
+  int64_t first;
+} mv_batch_tile;
+
+/* Per-item device state: draws[i] = iteration (Philox draw id) of item i;
+ * stats[4 i ..] = {last integral, last sigma, sum 1/sigma^2,
+ * sum integral/sigma^2} (the weighted average of monte_carlo.py:713-732 is
+ * stats[3]/stats[2], its error sqrt(1/stats[2])). */
+
+/* One iteration of the events listed in ``tiles`` (device array) for the
+ * items (device array): fused draws/map/integrand/sums/histograms as
+ * mv_vegas_fused, added with FP64 atomics into each item's row. ``blocks``
+ * persistent workgroups stride over the tiles. precision: mv_precision;
+ * family: the mv_family shared by every tiled item, or -1 for a mix;
+ * max_dim: the largest n_dim among the tiled items. */
+int mv_batch_fused(const mv_batch_item* items, int32_t n_items, const mv_batch_tile* tiles,
+                   int32_t n_tiles, int32_t precision, int32_t family, int32_t max_dim,
+                   const int32_t* draws, const double* grids, double* rows, int32_t blocks,
+                   void* stream);
+
+/* Iteration end for the items named in ``jobs`` (device int32 pairs
+ * {item, dim}: dim = -1 reduces the item's sums into stats (vflow.py:437-438,
+ * monte_carlo.py:718-720), stores {integral, sigma} at history[2 (item *
+ * max_iter + draws[item])] while draws[item] < max_iter (history nullable)
+ * and advances draws[item]; dim = j refines grid row j of the item from its
+ * histogram (vflow.py:129-211) when the item trains). Consumed row columns
+ * are zeroed. */
+int mv_batch_finalize(const mv_batch_item* items, const int32_t* jobs, int32_t n_jobs,
+                      double* grids, double* rows, int32_t* draws, double* stats,
+                      double* history, int32_t max_iter, void* stream);
 
 #ifdef __cplusplus
 }

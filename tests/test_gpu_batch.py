@@ -1,0 +1,95 @@
+"""Batched integrals (mv_batch_*): each item must reproduce a lone VegasFlow
+run with the same seed (the same Philox draws) up to summation order, and a
+captured HIP graph must reproduce eager rounds replay after replay.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+import torch
+
+import vegasflow_amd as vf
+from vegasflow_amd import autotune
+from vegasflow_amd.batch import BatchVegas, random_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def small_specs():
+    return [
+        (vf.Gaussian(0.2, [0.3, 0.5, 0.6]), 3, 20000, 101),
+        (vf.Product(), 2, 5000, 102),
+        (vf.GaussianMix(), 4, 30011, 103),
+        (vf.BreitWignerPower(), 6, 10000, 104),
+        (vf.Constant(3.0), 1, 1000, 105),
+        (vf.Gaussian(0.1, 0.5), 12, 70000, 106),
+        (vf.Gaussian(0.15, 0.45), 16, 9000, 107),
+    ]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batch_items_match_lone_runs(dtype):
+    specs = small_specs()
+    b = BatchVegas(specs, dtype=dtype, events_per_tile=2048, blocks_per_cu=2)
+    b.run(3, graph=False)
+    hist = b.history()
+    assert hist.shape == (len(specs), 3, 2)
+    for i, (fn, d, n, seed) in enumerate(specs):
+        lone = vf.VegasFlow(d, n, seed=seed, verbose=False, dtype=dtype)
+        lone.compile(fn)
+        lone.run_integration(3)
+        for k, (r, s, _) in enumerate(lone.history):
+            assert abs(hist[i, k, 0] - r) <= 1e-10 * abs(r) + 1e-300, (i, k)
+            assert abs(hist[i, k, 1] - s) <= 1e-7 * abs(s) + 1e-300, (i, k)
+        np.testing.assert_allclose(b.grid(i).cpu().numpy(), lone.divisions.cpu().numpy(),
+                                   rtol=1e-10, atol=1e-13)
+    np.testing.assert_array_equal(b.iterations(), 3)
+
+
+def test_batch_graph_replays_match_eager_rounds():
+    specs = small_specs()
+    eager = BatchVegas(specs, events_per_tile=4096, blocks_per_cu=4)
+    graph = BatchVegas(specs, events_per_tile=4096, blocks_per_cu=4)
+    graph.capture()
+    for k in range(4):
+        eager.step(graph=False)
+        graph.step(graph=True)
+        torch.cuda.synchronize()
+        # eager work and a synchronisation between replays
+        scratch = torch.ones(1000, device="cuda").sum()
+        assert float(scratch) == 1000.0
+        np.testing.assert_allclose(graph.last()[0], eager.last()[0], rtol=1e-12)
+        np.testing.assert_allclose(graph.last()[1], eager.last()[1], rtol=1e-9)
+        np.testing.assert_allclose(graph.grids.cpu().numpy(), eager.grids.cpu().numpy(), rtol=1e-11,
+                                   atol=1e-14)
+    np.testing.assert_array_equal(graph.iterations(), 4)
+    np.testing.assert_allclose(graph.results()[0], eager.results()[0], rtol=1e-12)
+
+
+def test_autotune_picks_and_caches_a_shape():
+    path = tempfile.mktemp(suffix=".json")
+    os.environ["MIVEGAS_TUNE_CACHE"] = path
+    try:
+        specs = random_batch(16, seed=5)
+        b1 = BatchVegas(specs)
+        assert b1.events_per_tile in autotune.EVENTS_PER_TILE
+        assert os.path.exists(path)
+        autotune._memory.clear()
+        b2 = BatchVegas(specs)
+        assert (b2.events_per_tile, b2.blocks) == (b1.events_per_tile, b1.blocks)
+    finally:
+        os.environ.pop("MIVEGAS_TUNE_CACHE", None)
+        if os.path.exists(path):
+            os.remove(path)
+
+
+def test_random_batch_integrals_agree_with_closed_forms():
+    specs = random_batch(48, seed=23, d_range=(2, 6), a_range=(0.1, 0.3))
+    b = BatchVegas(specs, events_per_tile=4096, blocks_per_cu=4)
+    b.run(8)
+    res, err = b.results()
+    exact = np.array([fn.exact(d) for fn, d, _, _ in specs])
+    pulls = (res - exact) / err
+    assert np.all(np.isfinite(pulls))
+    assert np.mean(np.abs(pulls) < 4) > 0.9

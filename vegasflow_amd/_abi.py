@@ -99,7 +99,27 @@ class Finalize(ctypes.Structure):
     ]
 
 
-STRUCTS = [Integrand, Step, Partials, Strata, Finalize]
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("n_dim", _i32),
+        ("train", _i32),
+        ("n_events", _i64),
+        ("seed", _u64),
+        ("grid_offset", _i64),
+        ("row_offset", _i64),
+        ("fn", Integrand),
+    ]
+
+
+class BatchTile(ctypes.Structure):
+    _fields_ = [
+        ("item", _i32),
+        ("count", _i32),
+        ("first", _i64),
+    ]
+
+
+STRUCTS = [Integrand, Step, Partials, Strata, Finalize, BatchItem, BatchTile]
 
 _P = ctypes.POINTER
 # name -> (restype, argtypes); the names are exactly those of include/mivegas.h
@@ -127,6 +147,8 @@ SIGNATURES = {
     "mv_vplus_finalize": (ctypes.c_int, [_i32, _P(Strata), _vp, _vp, _vp, _vp, _i32, _i32, _f64,
                                          _f64, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
                                          _vp]),
+    "mv_batch_fused": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
+    "mv_batch_finalize": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,142 @@
+// fused_event.hpp - the per-event body of the fused VegasFlow kernels and
+// their LDS tables, shared by the single-integral and the batched kernels.
+//
+// Per (dimension, bin) the LDS table holds the map frac -> y = P + Q frac of
+// the integrand coordinate (Integrand::fold, integration limits folded in)
+// and the Jacobian factor W = dx * 50 (vflow.py:73, 78).
+#pragma once
+
+#include <type_traits>
+
+#include "device.hpp"
+
+namespace mv {
+
+__device__ __forceinline__ int clamp_bin(int b) { return b < 0 ? 0 : (b > kBins - 1 ? kBins - 1 : b); }
+
+template <typename T, int FAM>
+__device__ __forceinline__ void build_table(const mv_integrand& fn, const double* __restrict__ grid,
+                                            int d, const double* xmin, const double* xdelta,
+                                            typename std::conditional<sizeof(T) == 8, double2, float2>::type* tab,
+                                            T* tw) {
+  const int nb = d * kBins;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const int j = i / kBins, b = i - j * kBins;
+    const double lo = grid[j * (kBins + 1) + b];
+    const double dx = grid[j * (kBins + 1) + b + 1] - lo;  // vflow.py:73
+    double P = lo, Q = dx;
+    if (xmin) {  // monte_carlo.py:273
+      P = xmin[j] + lo * xdelta[j];
+      Q = dx * xdelta[j];
+    }
+    Integrand<T, FAM>::fold(fn, j, P, Q);
+    tab[i].x = (T)P;
+    tab[i].y = (T)Q;
+    tw[i] = (T)(dx * kFBins);  // vflow.py:78
+  }
+}
+
+// One event, FP64 (vflow.py:87-126 + 33-83 + 389-430). Returns tmp = w f
+// and, when training, adds tmp^2 to the wave's histogram ``wh``.
+template <int FAM, bool TRAIN>
+__device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t c0, uint32_t c1,
+                                            uint32_t draw, Key key, int groups, int d,
+                                            const double2* tab, const double* tw, double* wh,
+                                            const double* inj, double wscale) {
+  F.begin();
+  double w = 1.0;
+  uint32_t pk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    if (g < groups) {
+      uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
+      if (!inj) {
+        h = philox(c0, c1, draw, (uint32_t)g, key);
+        l = philox(c0, c1, draw, (uint32_t)(groups + g), key);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = 4 * g + q;
+        if (j < d) {
+          const double u = inj ? inj[j] : draw_f64(word(h, q), word(l, q));
+          const double xn = kFBins * (1.0 - u);  // vflow.py:117
+          const int ind = clamp_bin((int)xn);    // vflow.py:67
+          const double2 t = tab[j * kBins + ind];
+          const double frac = xn - floor(xn);    // vflow.py:75
+          F.add_folded(j, fma(t.y, frac, t.x));  // vflow.py:76
+          w = w * tw[j * kBins + ind];           // vflow.py:78
+          if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
+        }
+      }
+    }
+  }
+  const double tmp = (w * wscale) * F.value();  // vflow.py:416
+  if (TRAIN) {
+    const double tmp2 = tmp * tmp;  // vflow.py:417
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (g < groups) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = 4 * g + q;
+          if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
+        }
+      }
+    }
+  }
+  return tmp;
+}
+
+// One event, mixed precision: FP32 map and integrand from the 32 high bits
+// of each draw, FP64 result. Bin and fraction come from the 32x32->64
+// product s * 50 with s = ~hi, i.e. xn = 50 (1 - u) in fixed point, which
+// keeps the bin in [0, 49].
+template <int FAM, bool TRAIN>
+__device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
+                                              uint32_t draw, Key key, int groups, int d,
+                                              const float2* tab, const float* tw, double* wh,
+                                              const uint32_t* inj, double wscale) {
+  F.begin();
+  float w = 1.0f;
+  uint32_t pk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    if (g < groups) {
+      uint4 h = make_uint4(0, 0, 0, 0);
+      if (!inj) h = philox(c0, c1, draw, (uint32_t)g, key);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = 4 * g + q;
+        if (j < d) {
+          const uint32_t hw = inj ? inj[j] : word(h, q);
+          const uint64_t p = (uint64_t)(~hw) * (uint64_t)kBins;
+          const int ind = (int)(p >> 32);
+          const float frac = (float)(((uint32_t)p) >> 8) * 0x1p-24f;
+          const float2 t = tab[j * kBins + ind];
+          F.add_folded(j, fmaf(t.y, frac, t.x));
+          w = w * tw[j * kBins + ind];
+          if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
+        }
+      }
+    }
+  }
+  // w * f in FP32; the 1/N (and limits) factor in FP64 so that its rounding
+  // is not a systematic bias
+  const double tmp = (double)(w * F.value()) * wscale;
+  if (TRAIN) {
+    const double tmp2 = tmp * tmp;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (g < groups) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = 4 * g + q;
+          if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
+        }
+      }
+    }
+  }
+  return tmp;
+}
+
+}  // namespace mv

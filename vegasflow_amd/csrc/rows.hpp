@@ -86,10 +86,9 @@ __device__ __forceinline__ void row_add(double* p, double v) { unsafeAtomicAdd(p
 // resulting row [sum, sum2, hist(hist_len)] into this workgroup's group row.
 // SUMS = false leaves columns 0 and 1 alone (VEGAS+ sums per hypercube).
 template <bool TRAIN, bool SUMS = true>
-__device__ __forceinline__ void finish_block(double acc, double acc2, const double* hist,
-                                             int hist_len, const RowsK& R) {
+__device__ __forceinline__ void finish_row(double acc, double acc2, const double* hist, int hist_len,
+                                           double* row) {
   __shared__ double red[2 * kWaves];
-  double* row = group_row(R);
   if (SUMS) {
     acc = wave_sum(acc);
     acc2 = wave_sum(acc2);
@@ -113,6 +112,12 @@ __device__ __forceinline__ void finish_block(double acc, double acc2, const doub
       if (h != 0.0) row_add(&row[2 + i], h);
     }
   }
+}
+
+template <bool TRAIN, bool SUMS = true>
+__device__ __forceinline__ void finish_block(double acc, double acc2, const double* hist,
+                                             int hist_len, const RowsK& R) {
+  finish_row<TRAIN, SUMS>(acc, acc2, hist, hist_len, group_row(R));
 }
 
 }  // namespace mv

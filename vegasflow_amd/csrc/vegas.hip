@@ -14,6 +14,7 @@
 
 #include "abi_util.hpp"
 #include "device.hpp"
+#include "fused_event.hpp"
 #include "refine.hpp"
 #include "rows.hpp"
 
@@ -24,13 +25,9 @@ std::string& last_error_slot() {
   return s;
 }
 
-__device__ __forceinline__ int clamp_bin(int b) { return b < 0 ? 0 : (b > kBins - 1 ? kBins - 1 : b); }
-
 // ---------------------------------------------------------------------------
-// fused VegasFlow, FP64 (vflow.py:87-126 + 33-83 + 389-430,
-// monte_carlo.py:249-275)
-// LDS: per (dim, bin) the map frac -> y = P + Q frac (Integrand::fold), the
-// Jacobian factor W = dx * 50, and kWaves private histograms.
+// fused VegasFlow (vflow.py:87-126 + 33-83 + 389-430, monte_carlo.py:249-275)
+// LDS: the map table of fused_event.hpp and kWaves private histograms.
 // ---------------------------------------------------------------------------
 template <int FAM, bool TRAIN>
 __global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand fn,
@@ -42,86 +39,27 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand 
   double2* tab = reinterpret_cast<double2*>(smem);
   double* tw = smem + 2 * nb;
   double* hist = tw + nb;
-  const bool limits = s.xmin != nullptr;
-  for (int i = threadIdx.x; i < nb; i += kBlock) {
-    const int j = i / kBins, b = i - j * kBins;
-    const double lo = grid[j * (kBins + 1) + b];
-    const double dx = grid[j * (kBins + 1) + b + 1] - lo;  // vflow.py:73
-    double P = lo, Q = dx;
-    if (limits) {  // monte_carlo.py:273
-      P = s.xmin[j] + lo * s.xdelta[j];
-      Q = dx * s.xdelta[j];
-    }
-    Integrand<double, FAM>::fold(fn, j, P, Q);
-    tab[i] = make_double2(P, Q);
-    tw[i] = dx * kFBins;  // vflow.py:78
-  }
+  build_table<double, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, tw);
   if (TRAIN)
     for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
   __syncthreads();
   double* wh = hist + (threadIdx.x >> 6) * nb;
-
   Integrand<double, FAM> F(fn);
   const double* inj = static_cast<const double*>(s.inject);
-  const double wscale = limits ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
+  const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
     const uint64_t ge = (uint64_t)(s.event_offset + i);
-    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
-    F.begin();
-    double w = 1.0;
-    uint32_t pk[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      if (g < s.groups) {
-        uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
-        if (!inj) {
-          h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-          l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int j = 4 * g + q;
-          if (j < d) {
-            const double u = inj ? inj[i * d + j] : draw_f64(word(h, q), word(l, q));
-            const double xn = kFBins * (1.0 - u);           // vflow.py:117
-            const int ind = clamp_bin((int)xn);             // vflow.py:67
-            const double2 t = tab[j * kBins + ind];
-            const double frac = xn - floor(xn);             // vflow.py:75
-            F.add_folded(j, fma(t.y, frac, t.x));           // vflow.py:76
-            w = w * tw[j * kBins + ind];                    // vflow.py:78
-            if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
-          }
-        }
-      }
-    }
-    const double tmp = (w * wscale) * F.value();  // vflow.py:416
-    const double tmp2 = tmp * tmp;                // vflow.py:417
+    const double tmp = event_f64<FAM, TRAIN>(F, (uint32_t)ge, (uint32_t)(ge >> 32), s.draw_id, s.key,
+                                             s.groups, d, tab, tw, wh, inj ? inj + i * d : nullptr,
+                                             wscale);
     acc += tmp;
-    acc2 += tmp2;
-    if (TRAIN) {
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        if (g < s.groups) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int j = 4 * g + q;
-            if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
-          }
-        }
-      }
-    }
+    acc2 += tmp * tmp;
   }
   finish_block<TRAIN>(acc, acc2, hist, nb, R);
 }
 
-// ---------------------------------------------------------------------------
-// fused VegasFlow, mixed precision: FP32 map and integrand from the 32 high
-// bits of each draw, FP64 sums and histograms.
-// Bin and fraction come from the 32x32->64 product s * 50 with s = ~hi,
-// i.e. xn = 50 * (1 - u) in fixed point, which keeps ind in [0, 49].
-// ---------------------------------------------------------------------------
 template <int FAM, bool TRAIN>
 __global__ __launch_bounds__(kBlock) void vegas_fused_mixed(StepK s, mv_integrand fn,
                                                             const double* __restrict__ grid,
@@ -129,78 +67,26 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_mixed(StepK s, mv_integran
   extern __shared__ double smem[];
   const int d = s.n_dim;
   const int nb = d * kBins;
-  float2* tab = reinterpret_cast<float2*>(smem);   // nb float2
+  float2* tab = reinterpret_cast<float2*>(smem);    // nb float2
   float* tw = reinterpret_cast<float*>(smem + nb);  // nb float (nb/2 doubles)
   double* hist = smem + nb + (nb + 1) / 2;
-  const bool limits = s.xmin != nullptr;
-  for (int i = threadIdx.x; i < nb; i += kBlock) {
-    const int j = i / kBins, b = i - j * kBins;
-    const double lo = grid[j * (kBins + 1) + b];
-    const double dx = grid[j * (kBins + 1) + b + 1] - lo;
-    double P = lo, Q = dx;
-    if (limits) {
-      P = s.xmin[j] + lo * s.xdelta[j];
-      Q = dx * s.xdelta[j];
-    }
-    Integrand<float, FAM>::fold(fn, j, P, Q);
-    tab[i] = make_float2((float)P, (float)Q);
-    tw[i] = (float)(dx * kFBins);
-  }
+  build_table<float, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, tw);
   if (TRAIN)
     for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
   __syncthreads();
   double* wh = hist + (threadIdx.x >> 6) * nb;
-
   Integrand<float, FAM> F(fn);
   const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
-  const double wscale = limits ? s.xjac * s.xdeltajac : s.xjac;
+  const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
     const uint64_t ge = (uint64_t)(s.event_offset + i);
-    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
-    F.begin();
-    float w = 1.0f;
-    uint32_t pk[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      if (g < s.groups) {
-        uint4 h = make_uint4(0, 0, 0, 0);
-        if (!inj) h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int j = 4 * g + q;
-          if (j < d) {
-            const uint32_t hw = inj ? inj[i * d + j] : word(h, q);
-            const uint64_t p = (uint64_t)(~hw) * (uint64_t)kBins;
-            const int ind = (int)(p >> 32);
-            const float frac = (float)(((uint32_t)p) >> 8) * 0x1p-24f;
-            const float2 t = tab[j * kBins + ind];
-            F.add_folded(j, fmaf(t.y, frac, t.x));
-            w = w * tw[j * kBins + ind];
-            if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
-          }
-        }
-      }
-    }
-    // w * f in FP32; the 1/N (and limits) factor in FP64 so that its
-    // rounding is not a systematic bias
-    const double tmp = (double)(w * F.value()) * wscale;
-    const double tmp2 = tmp * tmp;
+    const double tmp = event_mixed<FAM, TRAIN>(F, (uint32_t)ge, (uint32_t)(ge >> 32), s.draw_id, s.key,
+                                               s.groups, d, tab, tw, wh, inj ? inj + i * d : nullptr,
+                                               wscale);
     acc += tmp;
-    acc2 += tmp2;
-    if (TRAIN) {
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        if (g < s.groups) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int j = 4 * g + q;
-            if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
-          }
-        }
-      }
-    }
+    acc2 += tmp * tmp;
   }
   finish_block<TRAIN>(acc, acc2, hist, nb, R);
 }
@@ -535,6 +421,8 @@ extern "C" int64_t mv_struct_size(int32_t which) {
     case 2: return (int64_t)sizeof(mv_partials);
     case 3: return (int64_t)sizeof(mv_strata);
     case 4: return (int64_t)sizeof(mv_finalize);
+    case 5: return (int64_t)sizeof(mv_batch_item);
+    case 6: return (int64_t)sizeof(mv_batch_tile);
     default: return -1;
   }
 }

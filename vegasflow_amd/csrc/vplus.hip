@@ -11,6 +11,7 @@
 
 #include "abi_util.hpp"
 #include "device.hpp"
+#include "fused_event.hpp"
 #include "refine.hpp"
 #include "rows.hpp"
 
@@ -87,7 +88,6 @@ __device__ __forceinline__ void segment_add(int cube, double v, double v2, doubl
   }
 }
 
-__device__ __forceinline__ int clamp_bin(int b) { return b < 0 ? 0 : (b > kBins - 1 ? kBins - 1 : b); }
 
 
 // ---------------------------------------------------------------------------

@@ -271,3 +271,20 @@ def vplus_finalize(n_dim, strata, n_ev, offsets, ress, ress2, adaptive, min_neva
               _ptr(ws.rows), ws.stride, int(n_rows), int(bool(train)), int(bool(refine)),
               _ptr(grid), _ptr(out), _ptr(arr_var_out), _ptr(hist_out), _stream())
     return out
+
+
+# ---------------------------------------------------------------------------
+# batched integrals
+# ---------------------------------------------------------------------------
+def batch_fused(items_dev, n_items, tiles_dev, n_tiles, precision, family, max_dim, draws, grids, rows,
+                blocks):
+    """mv_batch_fused; every argument is already on the device (no
+    allocation here, so the call can be captured in a HIP graph)."""
+    _abi.call("mv_batch_fused", _ptr(items_dev), int(n_items), _ptr(tiles_dev), int(n_tiles),
+              precision_code(precision), int(family), int(max_dim), _ptr(draws), _ptr(grids), _ptr(rows),
+              int(blocks), _stream())
+
+
+def batch_finalize(items_dev, jobs, n_jobs, grids, rows, draws, stats, history=None, max_iter=0):
+    _abi.call("mv_batch_finalize", _ptr(items_dev), _ptr(jobs), int(n_jobs), _ptr(grids), _ptr(rows),
+              _ptr(draws), _ptr(stats), _ptr(history), int(max_iter), _stream())

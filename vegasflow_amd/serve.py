@@ -1,0 +1,252 @@
+"""Continuous-batching integration server on one GPU.
+
+Requests (a built-in integrand, n_dim, events per iteration, relative-error
+target, iteration cap, seed) are admitted into slots of a device slab whose
+size follows from a memory budget. Every tick advances all live slots by one
+iteration with the batched kernels (``mv_batch_fused`` per family present,
+then ``mv_batch_finalize``): requests that arrived while others run join at
+the next tick, and their work shares the same launches. After each tick the
+host reads the per-slot iteration counters and running weighted averages,
+retires the requests that reached their target (after ``min_iter``
+iterations) or their cap, and admits waiting requests into the freed slots.
+
+A request's draws depend only on its seed, its iteration and its event
+index, so its result is the same alone and in the mix up to summation order
+(``run_alone`` reproduces it with VegasFlow).
+"""
+import collections
+import ctypes
+import dataclasses
+import time
+
+import numpy as np
+import torch
+
+from . import _abi, kernels
+from .configflow import BINS_MAX, DTYPE, default_device, precision_of
+from .integrands import BuiltinIntegrand
+
+MAX_DIM = _abi.FUSED_MAX_DIM
+GRID_STRIDE = MAX_DIM * (BINS_MAX + 1)
+ROW_STRIDE = 2 + MAX_DIM * BINS_MAX
+ITEM_BYTES = ctypes.sizeof(_abi.BatchItem)
+TILE_BYTES = ctypes.sizeof(_abi.BatchTile)
+
+
+@dataclasses.dataclass
+class Request:
+    integrand: BuiltinIntegrand
+    n_dim: int
+    n_events: int
+    target: float = 1e-3
+    max_iter: int = 20
+    seed: int = 0
+    arrival: float = 0.0
+    rid: int = -1
+    # filled by the server
+    admitted: float = None
+    done: float = None
+    result: float = None
+    error: float = None
+    iterations: int = 0
+    slot: int = -1
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+def slot_bytes():
+    """Device bytes one slot holds (item, grid, row, counters)."""
+    return ITEM_BYTES + 8 * (GRID_STRIDE + ROW_STRIDE) + 4 + 8 * 4
+
+
+class Server:
+    """Request loop over a slab of ``capacity`` slots (from memory_budget)."""
+
+    def __init__(self, memory_budget=64 << 20, dtype=None, min_iter=3, events_per_tile=4096,
+                 blocks_per_cu=4):
+        kernels.require_gpu()
+        self.device = default_device()
+        self.precision = precision_of(DTYPE if dtype is None else dtype)
+        self.capacity = int(memory_budget // slot_bytes())
+        if self.capacity < 1:
+            raise ValueError("memory budget below one slot")
+        self.min_iter = int(min_iter)
+        self.events_per_tile = int(events_per_tile)
+        self.blocks = int(blocks_per_cu) * torch.cuda.get_device_properties(self.device).multi_processor_count
+        S = self.capacity
+        self.items_host = (_abi.BatchItem * S)()
+        self.items = torch.zeros(S * ITEM_BYTES, dtype=torch.uint8, device=self.device)
+        self.grids = torch.zeros(S * GRID_STRIDE, dtype=torch.float64, device=self.device)
+        self.rows = torch.zeros(S * ROW_STRIDE, dtype=torch.float64, device=self.device)
+        self.draws = torch.zeros(S, dtype=torch.int32, device=self.device)
+        self.stats = torch.zeros(4 * S, dtype=torch.float64, device=self.device)
+        self._linspace = torch.linspace(0, 1, BINS_MAX + 1, dtype=torch.float64, device=self.device)
+        self.free = list(range(S - 1, -1, -1))
+        self.live = {}  # slot -> Request
+        self.queue = collections.deque()
+        self.finished = []
+        self.ticks = 0
+        self.events_done = 0
+        self._tile_cache = {}
+
+    # ---------------------------------------------------------- admission
+    def submit(self, req):
+        if not isinstance(req.integrand, BuiltinIntegrand) or not req.integrand.fused_ok(req.n_dim) \
+                or req.n_dim > MAX_DIM:
+            raise ValueError(f"requests need a built-in integrand with n_dim <= {MAX_DIM}")
+        if req.n_events < 2:
+            raise ValueError("a request needs at least 2 events per iteration")
+        self.queue.append(req)
+
+    def _admit(self, req, now):
+        s = self.free.pop()
+        it = self.items_host[s]
+        it.n_dim = req.n_dim
+        it.train = 1
+        it.n_events = req.n_events
+        it.seed = req.seed & 0xFFFFFFFFFFFFFFFF
+        it.grid_offset = s * GRID_STRIDE
+        it.row_offset = s * ROW_STRIDE
+        it.fn = req.integrand.to_struct(req.n_dim)
+        raw = torch.frombuffer(bytearray(bytes(it)), dtype=torch.uint8)
+        self.items[s * ITEM_BYTES:(s + 1) * ITEM_BYTES].copy_(raw)
+        g = self.grids[s * GRID_STRIDE:s * GRID_STRIDE + req.n_dim * (BINS_MAX + 1)]
+        g.view(req.n_dim, BINS_MAX + 1).copy_(self._linspace.expand(req.n_dim, BINS_MAX + 1))
+        self.draws[s] = 0
+        self.stats[4 * s:4 * s + 4] = 0.0
+        req.slot = s
+        req.admitted = now
+        self.live[s] = req
+
+    def _tiles_of(self, s, req):
+        key = (s, req.n_events)
+        t = self._tile_cache.get(key)
+        if t is None:
+            n = req.n_events
+            firsts = np.arange(0, n, self.events_per_tile, dtype=np.int64)
+            counts = np.minimum(self.events_per_tile, n - firsts).astype(np.int32)
+            t = np.zeros(len(firsts), dtype=[("item", "<i4"), ("count", "<i4"), ("first", "<i8")])
+            t["item"] = s
+            t["count"] = counts
+            t["first"] = firsts
+            self._tile_cache[key] = t
+        return t
+
+    # --------------------------------------------------------------- tick
+    def tick(self):
+        """One iteration for every live slot; returns the requests retired."""
+        if not self.live:
+            return []
+        by_family = collections.defaultdict(list)
+        jobs = []
+        for s, req in self.live.items():
+            by_family[req.integrand.family].append((s, req))
+            jobs.append((s, -1))
+            jobs.extend((s, j) for j in range(req.n_dim))
+        tensors = []
+        for fam, members in sorted(by_family.items()):
+            tiles = np.concatenate([self._tiles_of(s, r) for s, r in members])
+            tiles_dev = torch.from_numpy(tiles.view(np.uint8)).to(self.device, non_blocking=False)
+            tensors.append(tiles_dev)
+            max_dim = max(r.n_dim for _, r in members)
+            kernels.batch_fused(self.items, self.capacity, tiles_dev, len(tiles), self.precision, fam, max_dim,
+                                self.draws, self.grids, self.rows, self.blocks)
+            self.events_done += sum(r.n_events for _, r in members)
+        jobs_dev = torch.tensor(np.array(jobs, dtype=np.int32).reshape(-1), device=self.device)
+        kernels.batch_finalize(self.items, jobs_dev, len(jobs), self.grids, self.rows, self.draws, self.stats)
+        st = self.stats.view(-1, 4).cpu().numpy()  # synchronises
+        draws = self.draws.cpu().numpy()
+        now = time.perf_counter()
+        self.ticks += 1
+        retired = []
+        for s, req in list(self.live.items()):
+            k = int(draws[s])
+            with np.errstate(divide="ignore", invalid="ignore"):
+                est = st[s, 3] / st[s, 2]
+                err = np.sqrt(1.0 / st[s, 2])
+            req.iterations = k
+            req.result, req.error = float(est), float(err)
+            converged = k >= self.min_iter and np.isfinite(err) and err <= req.target * abs(est)
+            if converged or k >= req.max_iter:
+                req.done = now
+                retired.append(req)
+                del self.live[s]
+                self.free.append(s)
+        self.finished.extend(retired)
+        return retired
+
+    def admit_waiting(self, now):
+        while self.queue and self.free and self.queue[0].arrival <= now:
+            self._admit(self.queue.popleft(), now)
+
+    # -------------------------------------------------------------- loops
+    def run_stream(self, requests, clock=time.perf_counter):
+        """Serve ``requests`` (sorted by ``arrival``, seconds from the
+        start) in real time; returns the finished requests."""
+        t0 = clock()
+        for r in requests:
+            r.arrival = t0 + r.arrival
+            self.submit(r)
+        while self.queue or self.live:
+            now = clock()
+            self.admit_waiting(now)
+            if self.live:
+                self.tick()
+            elif self.queue:
+                wait = self.queue[0].arrival - clock()
+                if wait > 0:
+                    time.sleep(min(wait, 0.001))
+        return self.finished
+
+
+def poisson_requests(rate=200.0, duration=60.0, seed=5, events_gauss=(1_000_000, 20_000_000),
+                     events_bw=(1_000_000, 20_000_000), targets=(1e-3, 1e-4), max_iter=20):
+    """The serving workload: Poisson arrivals at ``rate`` per second for
+    ``duration`` seconds; each request is the two-Gaussian mixture in d = 4
+    or the Breit-Wigner x power law in d = 6, with a relative-error target
+    drawn from ``targets`` (events per iteration: the first entry of the
+    events pair for the looser target, the second for the tighter one)."""
+    from .integrands import BreitWignerPower, GaussianMix
+
+    rng = np.random.default_rng(seed)
+    t, out, rid = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        target = float(targets[int(rng.integers(len(targets)))])
+        tight = target == min(targets)
+        if rng.random() < 0.5:
+            fn, d, n = GaussianMix(), 4, events_gauss[1 if tight else 0]
+        else:
+            fn, d, n = BreitWignerPower(), 6, events_bw[1 if tight else 0]
+        out.append(Request(fn, d, n, target=target, max_iter=max_iter, seed=10_000 + rid, arrival=t, rid=rid))
+        rid += 1
+    return out
+
+
+def run_alone(req, dtype=None, n_iter=None):
+    """The request on its own through VegasFlow: the same draws, iterations
+    ``n_iter`` (default: what it ran in the server)."""
+    from .vflow import VegasFlow
+
+    inst = VegasFlow(req.n_dim, req.n_events, seed=req.seed, verbose=False,
+                     dtype=DTYPE if dtype is None else dtype)
+    inst.compile(req.integrand)
+    return inst.run_integration(n_iter or req.iterations)
+
+
+def latency_report(finished):
+    lat = np.array([r.latency for r in finished if r.latency is not None])
+    if lat.size == 0:
+        return {}
+    return {
+        "requests": int(lat.size),
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "max_ms": float(lat.max() * 1e3),
+        "mean_iterations": float(np.mean([r.iterations for r in finished])),
+        "met_target": float(np.mean([r.error <= r.target * abs(r.result) for r in finished])),
+    }

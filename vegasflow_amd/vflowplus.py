@@ -77,6 +77,8 @@ class VegasFlowPlus(VegasFlow):
         self._ress = torch.zeros(n_cubes, dtype=torch.float64, device=self.device)
         self._ress2 = torch.zeros(n_cubes, dtype=torch.float64, device=self.device)
         self._last_out = None
+        self._pending_events = []
+        self.events_history = []  # events of each iteration run so far
         if self._adaptive:
             logger.warning("Variable number of events requires function signatures all across")
 
@@ -193,6 +195,7 @@ class VegasFlowPlus(VegasFlow):
 
     def _iteration_content(self):
         """vflowplus.py:222-242"""
+        self._pending_events.append(self._current_events())
         self._next_draw()
         ress, arr_var, arr_res2 = self.run_event(n_ev=self._n_ev)
         sigmas2 = torch.clamp(arr_var, min=0.0)
@@ -215,11 +218,15 @@ class VegasFlowPlus(VegasFlow):
         out = kernels.vplus_finalize(self.n_dim, strata, self._n_ev, self._offsets, self._ress, self._ress2,
                                      self._adaptive, self.min_neval_hcube, self._init_calls, BETA, ws, ng,
                                      self.train, self.train, self.divisions)
+        self._pending_events.append(self._last_out[2] if self._last_out is not None else self._n_events)
         self._last_out = out
         return out[0], out[1]
 
     def run_integration(self, n_iter, log_time=True, histograms=None):
         result = super().run_integration(n_iter, log_time=log_time, histograms=histograms)
+        for e in self._pending_events:
+            self.events_history.append(int(e.item()) if torch.is_tensor(e) else int(e))
+        self._pending_events = []
         if self._last_out is not None:
             self._n_events = int(self._last_out[2].item())
             self._last_out = None
