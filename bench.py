@@ -82,7 +82,65 @@ def protocol(vf, dtype, fn, d, n, seed):
     inst.freeze_grid()
     res, sig = inst.run_integration(5)
     torch.cuda.synchronize()
-    return float(res), float(sig)
+    return float(res), float(sig), float(inst.chi2_dof)
+
+
+def unfused_bytes(d, s):
+    """Algorithmic HBM bytes per event of the unfused path (float size s,
+    uint8 bins): sample writes x, w, bins; digest reads f, w, bins."""
+    return {"sample": s * d + s + d, "digest": 2 * s + d}
+
+
+def bench_unfused(vf, dtype, steps=10):
+    """The torch-hook path at the headline shape: VegasFlow with the Gaussian
+    written as a torch function (not a built-in), d=8, 1e7 events per
+    iteration in one step; the sample and digest kernels are timed with HIP
+    events around each launch and priced against HBM bandwidth."""
+    import torch
+    from vegasflow_amd import kernels
+
+    s = 8 if dtype == torch.float64 else 4
+
+    def gaussian(x, weight=None):
+        pref = (1.0 / (A * math.sqrt(math.pi))) ** x.shape[1]
+        return pref * torch.exp(-torch.sum(torch.square(x - 0.5), dim=1) / A ** 2)
+
+    inst = vf.VegasFlow(D, N, seed=SEED, verbose=False, dtype=dtype, events_limit=N)
+    inst.compile(gaussian)
+    stream = torch.cuda.current_stream()
+    ws = inst._workspace_for(kernels.partial_stride(D, 1, True))
+    x = torch.empty((N, D), dtype=dtype, device="cuda")
+    w = torch.empty((N,), dtype=dtype, device="cuda")
+    ind = torch.empty((N, D), dtype=torch.uint8, device="cuda")
+    marks = []
+    for k in range(steps + 2):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        inst._next_draw()
+        step = inst._step(N, train=True, offset=0)
+        ev[0].record(stream)
+        kernels.vegas_sample(step, inst.divisions, x, w, ind)
+        ev[1].record(stream)
+        f = gaussian(x)
+        ev[2].record(stream)
+        ng = kernels.vegas_digest(step, f, w, ind, ws)
+        ev[3].record(stream)
+        kernels.vegas_finalize(D, 1, True, True, N, ws, ng, grid=inst.divisions)
+        if k >= 2:
+            marks.append(ev)
+    torch.cuda.synchronize()
+    ms = [sum(m[i].elapsed_time(m[i + 1]) for m in marks) / steps for i in range(3)]
+    b = unfused_bytes(D, s)
+    t0 = time.perf_counter()
+    res, sig = inst.run_integration(5)
+    dt = time.perf_counter() - t0
+    return {"events_per_s_run_integration": 5 * N / dt, "integral_5_iterations": float(res),
+            "sigma": float(sig),
+            "sample_kernel": {"ms": ms[0], "bytes_per_event": b["sample"],
+                              "achieved_GB_s": b["sample"] * N / (ms[0] * 1e-3) / 1e9},
+            "torch_integrand_ms": ms[1],
+            "digest_kernel": {"ms": ms[2], "bytes_per_event": b["digest"],
+                              "achieved_GB_s": b["digest"] * N / (ms[2] * 1e-3) / 1e9},
+            "hbm_peak_GB_s": 8000.0}
 
 
 def bench_vplus(vf, dtype, n_iter=10):
@@ -102,7 +160,7 @@ def bench_vplus(vf, dtype, n_iter=10):
     dt = time.perf_counter() - t0
     events = sum(inst.events_history)
     return {"integral": float(res), "sigma": float(sig), "pull_vs_exact": (float(res) - fn.exact(4)) / float(sig),
-            "events_per_s": events / dt, "ms_per_iteration": 1e3 * dt / n_iter, "events": events,
+            "chi2_dof": float(inst.chi2_dof), "events_per_s": events / dt, "ms_per_iteration": 1e3 * dt / n_iter, "events": events,
             "iterations": n_iter, "events_first_iteration": inst.events_history[0],
             "events_last_iteration": inst.events_history[-1]}
 
@@ -129,7 +187,8 @@ def bench_batched(torch, dtype, n_iter=8):
             "events_per_round": b.events_per_round(), "items": len(specs),
             "events_per_tile": b.events_per_tile, "workgroups": b.blocks,
             "frac_within_3sigma": float(np.mean(np.abs(pulls) < 3)),
-            "median_abs_pull": float(np.median(np.abs(pulls)))}
+            "median_abs_pull": float(np.median(np.abs(pulls))),
+            "median_chi2_dof": float(np.nanmedian(b.chi2_dof()))}
 
 
 def bench_serving(seconds, rate=200.0):
@@ -150,35 +209,101 @@ def bench_serving(seconds, rate=200.0):
     return rep
 
 
-def cpu_baseline(seconds_budget=20.0):
-    """The NumPy FP64 oracle (a restatement of the reference algorithm,
-    Philox draws included) timed on the host on one core-bound iteration."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(numpy_seconds=6.0):
+    """The reference algorithm on the host cores, FP64.
+
+    Headline: the reference's TF graph restated op for op with torch CPU
+    tensors (dense one-hot histograms, 1e6-event steps, serial rebin),
+    multi-threaded, timed on one whole training iteration of the benchmark
+    workload (d=8 Gaussian, 1e7 events). Beside it: the NumPy oracle,
+    single-threaded, on 2e5-event iterations for ``numpy_seconds``, and the
+    reference's own CPU case (d=4 Gaussian, 5 x 1e5, seed 7) with its result.
+    """
     import numpy as np
+    import torch
+
+    from oracle import torch_cpu as tc
     from oracle import vegas as ora
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    torch.set_num_threads(threads)
+    gen = torch.Generator().manual_seed(SEED)
+    div = torch.from_numpy(np.ascontiguousarray(ora.initial_grid(D)))
+    t0 = time.perf_counter()
+    _, _, done = tc.iteration(div, N, tc.gaussian, gen)
+    t_torch = time.perf_counter() - t0
+
+    gen = torch.Generator().manual_seed(7)
+    div4 = torch.from_numpy(np.ascontiguousarray(ora.initial_grid(4)))
+    t0 = time.perf_counter()
+    its = [tc.iteration(div4, 100_000, tc.gaussian, gen)[:2] for _ in range(5)]
+    t_c1 = time.perf_counter() - t0
+    c1_res, c1_sig = ora.weighted_average(its)
 
     n = 200_000
     grid = ora.initial_grid(D)
     fn = lambda x: ora.gaussian(x, A, 0.5)  # noqa: E731
-    reps, t_total = 0, 0.0
-    while t_total < seconds_budget and reps < 50:
+    reps, t_np = 0, 0.0
+    while t_np < numpy_seconds and reps < 50:
         t0 = time.perf_counter()
         u = ora.uniform_f64(SEED, reps, n, D)
         ora.vegas_iteration(grid, u, fn, n)
-        t_total += time.perf_counter() - t0
+        t_np += time.perf_counter() - t0
         reps += 1
     return {
-        "value": reps * n / t_total,
+        "value": done / t_torch,
         "unit": "events/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{reps} training iterations x {n} events, d=8 Gaussian, NumPy FP64 oracle "
-                  f"(Philox draws, grid map, bin histograms, refinement), single-threaded",
+        "sample": f"one training iteration of {done} events (10 steps of 1e6), d=8 Gaussian, FP64, "
+                  f"torch-CPU op-for-op restatement of the reference graph, {threads} threads, "
+                  f"{t_torch:.1f} s",
+        "cpu_model": _cpu_model(),
+        "numpy_oracle_1_thread": {"events_per_s": reps * n / t_np,
+                                  "sample": f"{reps} training iterations x {n} events, d=8 Gaussian, "
+                                            f"Philox draws included"},
+        "reference_cpu_case_d4": {"integral": float(c1_res), "sigma": float(c1_sig),
+                                  "pull_vs_exact": (float(c1_res) - 0.99999999999385025) / float(c1_sig),
+                                  "events_per_s": 5 * 100_000 / t_c1,
+                                  "sample": "5 training iterations x 1e5 events, d=4 Gaussian a=0.1"},
     }
+
+
+def calibrate_copy(torch, nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of ``nbytes`` (torch's copy kernel):
+    the reference point that scales the FETCH_SIZE / WRITE_SIZE counters to
+    bytes, and the achievable HBM bandwidth of this GPU."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    return {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": reps + 1,
+            "ms_per_copy": ms, "GB_per_s_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only time a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -196,6 +321,9 @@ def main():
     import vegasflow_amd as vf
 
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        print(json.dumps(calibrate_copy(torch)))
+        return 0
     fn = vf.Gaussian(A, 0.5)
     exact = fn.exact(D)
 
@@ -213,13 +341,14 @@ def main():
     err = {}
     if not args.no_protocol:
         for prec, dtype in (("mixed", torch.float32), ("f64", torch.float64)):
-            res, sig = protocol(vf, dtype, fn, D, N, SEED)
-            err[prec] = {"integral": res, "sigma": sig, "pull_vs_exact": (res - exact) / sig}
+            res, sig, chi2 = protocol(vf, dtype, fn, D, N, SEED)
+            err[prec] = {"integral": res, "sigma": sig, "pull_vs_exact": (res - exact) / sig,
+                         "chi2_dof_frozen": chi2}
         prod = vf.Product()
         for prec, dtype in (("mixed", torch.float32), ("f64", torch.float64)):
-            res, sig = protocol(vf, dtype, prod, D, N, SEED + 1)
+            res, sig, chi2 = protocol(vf, dtype, prod, D, N, SEED + 1)
             err[prec + "_product"] = {"integral": res, "sigma": sig,
-                                      "pull_vs_exact": (res - 2.0 ** -D) / sig}
+                                      "pull_vs_exact": (res - 2.0 ** -D) / sig, "chi2_dof_frozen": chi2}
         err["mixed_minus_f64_in_sigma"] = (err["mixed"]["integral"] - err["f64"]["integral"]) / err["f64"]["sigma"]
 
     head = results["mixed"]
@@ -261,6 +390,8 @@ def main():
     }
     line["roofline"]["f64_kernel"]["frac"] = line["roofline"]["f64_kernel"]["achieved"] / PEAK["f64"]
     if not args.no_extra:
+        line["unfused_torch_integrand"] = {p: bench_unfused(vf, dt_) for p, dt_ in
+                                           (("mixed", torch.float32), ("f64", torch.float64))}
         line["vegasflowplus_d4"] = {p: bench_vplus(vf, dt_) for p, dt_ in
                                     (("mixed", torch.float32), ("f64", torch.float64))}
         line["batched_256"] = {p: bench_batched(torch, dt_) for p, dt_ in

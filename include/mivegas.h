@@ -138,8 +138,9 @@ const char* mv_last_error(void);
 int64_t mv_struct_size(int32_t which);
 /* Doubles per partial row: 2 * n_out + (train ? n_dim * MV_BINS : 0). */
 int64_t mv_partial_stride(int32_t n_dim, int32_t n_out, int32_t train);
-/* Workgroups a producer launches for ``n_events`` events, and the group
- * rows it then touches. */
+/* Upper bound of the workgroups a producer launches for ``n_events``
+ * events (the launch also caps them at what is resident on the device at
+ * once), and the group rows it then touches. */
 int32_t mv_producer_blocks(int64_t n_events);
 int32_t mv_producer_rows(int64_t n_events);
 
@@ -167,7 +168,8 @@ int mv_vegas_digest(const mv_step* step, const void* f, int64_t f_stride, int32_
                     int32_t main_dim, const void* w, int64_t w_stride, const uint8_t* idx,
                     const mv_partials* parts, void* stream);
 
-/* Reduce ``n_rows`` group rows (fixed order, then zero them) into
+/* Reduce ``n_rows`` (1..MV_ROWS, as returned by the producer) group rows
+ * (fixed order, then zero them) into
  * out[0:n_out] = res and
  * out[n_out:2 n_out] = sigma (vflow.py:437-438, plain.py:37-43),
  * hist_out (nullable, (n_dim, MV_BINS)) = arr_res2, and when fz->refine
@@ -245,7 +247,11 @@ typedef struct {
   mv_integrand fn;
 } mv_batch_item;
 
-/* A tile: ``count`` consecutive events of one item, from event ``first``. */
+/* A tile: ``count`` consecutive events of one item, from event ``first``.
+ * A tile must not cross a multiple of 2^32 events (first mod 2^32 + count
+ * <= 2^32): the kernel takes the high Philox counter word of its first event
+ * for the whole tile. Tiles cut at multiples of a power-of-two size never
+ * cross one. */
 typedef struct {
   int32_t item;
   int32_t count;

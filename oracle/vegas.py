@@ -70,6 +70,33 @@ def hi_words(seed, draw_id, n, n_dim, event_offset=0):
     return words[:, :n_dim].copy()
 
 
+MIXED_P0 = 50 * 2 ** 23 - 25
+
+
+def mixed_position(words):
+    """Bin and fraction of the mixed-precision path (the engine's own form;
+    the reference has only FP64/FP32 uniforms): the leading 23 bits
+    s = word >> 9 give xn = 50 (1 - (s + 1/2) / 2^23), held exactly as
+    p = xn 2^23 = 50 2^23 - 25 - 50 s. Returns (bin = p >> 23,
+    frac = (p mod 2^23) / 2^23), i.e. xn = bin + frac exactly."""
+    s = np.asarray(words).astype(np.int64) >> 9
+    p = MIXED_P0 - 50 * s
+    return (p >> 23).astype(np.int32), (p & (2 ** 23 - 1)).astype(np.float64) * 2.0 ** -23
+
+
+def mixed_map(words, divisions):
+    """x (n, d), w (n,), ind (n, d) of the mixed path on ``words``, evaluated
+    in FP64 (the GPU evaluates the same positions in FP32)."""
+    ind, frac = mixed_position(words)
+    d = divisions.shape[0]
+    cols = np.arange(d)[None, :]
+    lo = divisions[cols, ind]
+    dx = divisions[cols, ind + 1] - lo
+    x = lo + dx * frac
+    w = np.prod(dx * FBINS, axis=1)
+    return x, w, ind
+
+
 # ----------------------------------------------------------------------------
 # grid
 # ----------------------------------------------------------------------------

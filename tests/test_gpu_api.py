@@ -304,3 +304,29 @@ def test_consume_array_into_indices_api():
     for val, i in zip(input_array, indices):
         check[i] += val
     np.testing.assert_allclose(check, res.cpu().numpy(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("alg", [PlainFlow, VegasFlow])
+def test_histograms_hook_weighted_accumulation(alg):
+    """run_integration(histograms=...) (monte_carlo.py:645-741, as in
+    examples/histogram_ex.py): each iteration's histogram is kept and the
+    variable ends as the inverse-variance weighted average, so its bins sum
+    to the final integral when the integrand fills w * f."""
+    hist = torch.zeros(2, dtype=torch.float64, device="cuda")
+
+    def integrand(xarr, weight=None):
+        res = example_integrand(xarr)
+        b = torch.clamp((xarr[:, 1] * 2).long(), 0, 1)
+        hist.index_add_(0, b, (res * weight).to(torch.float64))
+        return res
+
+    inst = alg(3, 100_000, seed=4, verbose=False)
+    inst.compile(integrand)
+    hist.zero_()  # the compile-time shape check ran the integrand
+    res, err = inst.run_integration(3, histograms=(hist,))
+    assert abs(float(hist.sum()) - res) <= 1e-9 * abs(res)
+    per_it = [h[0] for _, _, h in inst.history[-3:]]
+    for (r, _, _), h in zip(inst.history[-3:], per_it):
+        assert abs(float(h.sum()) - r) <= 1e-9 * abs(r)
+    assert abs(float(hist[0]) - float(hist[1])) < 5 * err + 0.05 * res  # symmetric integrand
+    assert np.isfinite(inst.chi2_dof) and inst.chi2_dof >= 0

@@ -158,6 +158,53 @@ def _fused_once(fam, d, n, grid, train, seed=7, draw=0, limits=None, precision="
     return out.cpu().numpy(), (hist.cpu().numpy() if train else None), g.cpu().numpy()
 
 
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_fused_event_range_across_a_2_32_counter_boundary(precision):
+    """A step whose event counters cross a multiple of 2^32 is cut into
+    launches with a uniform high counter word; the draws stay those of the
+    oracle's 64-bit event counter."""
+    d, n, off = 5, 20000, (1 << 32) - 7001
+    grid = random_grid(np.random.default_rng(7), d)
+    fn, ofn = FAMILIES["gauss_offc"]
+    step = kernels.make_step(d, n, precision=precision, train=True, event_offset=off, seed=5, draw_id=9,
+                             xjac=1.0 / n)
+    ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, True))
+    ng = kernels.vegas_fused(step, fn.to_struct(d), dev(grid), ws)
+    hist = torch.empty((d, 50), dtype=torch.float64, device=DEV)
+    out = kernels.vegas_finalize(d, 1, True, False, n, ws, ng, hist_out=hist).cpu().numpy()
+    if precision == "f64":
+        x, w, ind = ora.vegas_map(ora.uniform_f64(5, 9, n, d, event_offset=off), grid)
+        tol = RTOL
+    else:
+        x, w, ind = ora.mixed_map(ora.hi_words(5, 9, n, d, event_offset=off), grid)
+        tol = 1e-5
+    x, w = ora.apply_xjac_and_limits(x, w, 1.0 / n)
+    res, res2, arr = ora.vegas_event(x, w, ind, ofn(x))
+    assert rel(out[0], res) < tol
+    np.testing.assert_allclose(hist.cpu().numpy(), arr, rtol=max(tol, 1e-12) * 10, atol=1e-6 * arr.max())
+    # plain MC across the same boundary
+    pstep = kernels.make_step(d, n, precision=precision, train=False, event_offset=off, seed=5, draw_id=9,
+                              xjac=1.0 / n)
+    pws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, False))
+    png = kernels.plain_fused(pstep, fn.to_struct(d), pws)
+    pout = kernels.vegas_finalize(d, 1, False, False, n, pws, png).cpu().numpy()
+    if precision == "f64":
+        pres, _ = ora.plain_iteration(ora.uniform_f64(5, 9, n, d, event_offset=off), ofn, n)
+        assert rel(pout[0], pres) < RTOL
+        # the unfused sampler across the boundary, bit for bit
+        xs = torch.empty((n, d), dtype=torch.float64, device=DEV)
+        ws_ = torch.empty(n, dtype=torch.float64, device=DEV)
+        idx = torch.empty((n, d), dtype=torch.uint8, device=DEV)
+        kernels.vegas_sample(step, dev(grid), xs, ws_, idx)
+        np.testing.assert_array_equal(idx.cpu().numpy(), ind)
+        np.testing.assert_array_equal(xs.cpu().numpy(), x)
+        np.testing.assert_array_equal(ws_.cpu().numpy(), w)
+    else:
+        u = (ora.hi_words(5, 9, n, d, event_offset=off) >> 9).astype(np.float64) * 2.0 ** -23 + 2.0 ** -24
+        pres, _ = ora.plain_iteration(u, ofn, n)
+        assert rel(pout[0], pres) < 1e-5
+
+
 @pytest.mark.parametrize("fam", list(FAMILIES))
 @pytest.mark.parametrize("d", [2, 4, 8])
 def test_fused_f64_iteration_matches_oracle(fam, d):
@@ -332,6 +379,44 @@ def test_plain_fused_matches_oracle(fam):
 
 
 # ----------------------------------------------------------------- mixed
+@pytest.mark.parametrize("d", [1, 5, 8, 17])
+def test_vegas_sample_mixed_matches_oracle_positions(d):
+    """Mixed sampling: bins bit-exact with oracle.mixed_position on the same
+    Philox words; x and w equal the FP64 evaluation of the same positions to
+    FP32 rounding."""
+    rng = np.random.default_rng(100 + d)
+    grid = random_grid(rng, d)
+    n, seed = 6000, 77
+    step = kernels.make_step(d, n, precision="mixed", train=True, seed=seed, draw_id=1, xjac=1.0 / n)
+    x = torch.empty((n, d), dtype=torch.float32, device=DEV)
+    w = torch.empty(n, dtype=torch.float32, device=DEV)
+    idx = torch.empty((n, d), dtype=torch.uint8, device=DEV)
+    kernels.vegas_sample(step, dev(grid), x, w, idx)
+    ox, ow, oind = ora.mixed_map(ora.hi_words(seed, 1, n, d), grid)
+    np.testing.assert_array_equal(idx.cpu().numpy(), oind)
+    np.testing.assert_allclose(x.cpu().numpy(), ox, rtol=2e-6, atol=1e-7)
+    np.testing.assert_allclose(w.cpu().numpy(), ow / n, rtol=4e-6 * d)
+    # never exactly at the lower edge; the top edge can round to 1.0f in FP32
+    assert x.min().item() > 0.0 and x.max().item() <= 1.0
+
+
+def test_mixed_fused_matches_oracle_positions():
+    """The fused mixed iteration equals the FP64 oracle evaluated on the
+    mixed positions (same words) to FP32 accuracy; bins are exact, so the
+    histogram counts agree."""
+    d, n = 6, 300000
+    rng = np.random.default_rng(12)
+    grid = random_grid(rng, d)
+    out, hist, _ = _fused_once("gauss_offc", d, n, grid, True, seed=31, precision="mixed")
+    x, w, ind = ora.mixed_map(ora.hi_words(31, 0, n, d), grid)
+    x, w = ora.apply_xjac_and_limits(x, w, 1.0 / n)
+    res, res2, arr = ora.vegas_event(x, w, ind, ora.gaussian(x, 0.2, 0.37))
+    sig = ora.vegas_sigma(res, res2, n)
+    assert rel(out[0], res) < 1e-5
+    assert rel(out[1], sig) < 1e-4
+    np.testing.assert_allclose(hist, arr, rtol=1e-4, atol=1e-6 * arr.max())
+
+
 def test_mixed_fused_tracks_f64_on_same_draws():
     """The mixed kernel samples the leading 32 bits of each FP64 draw: its
     integral differs from the FP64 one by FP32 rounding only."""
@@ -345,12 +430,11 @@ def test_mixed_fused_tracks_f64_on_same_draws():
 
 
 def test_mixed_fused_bins_match_integer_oracle():
-    """Mixed bins are floor(50 * ~hi / 2^32): compare the histogram of a
-    constant integrand (counts per bin) with the oracle's integer bins."""
+    """Mixed bins are those of oracle.mixed_position: compare the histogram
+    of a constant integrand (counts per bin) with the oracle's integer bins."""
     d, n = 3, 200000
     out, hist, _ = _fused_once("const", d, n, ora.initial_grid(d), True, seed=4, precision="mixed")
-    words = ora.hi_words(4, 0, n, d).astype(np.uint64)
-    bins = ((np.uint64(0xFFFFFFFF) - words) * np.uint64(50)) >> np.uint64(32)
+    bins, _ = ora.mixed_position(ora.hi_words(4, 0, n, d))
     tmp2 = (2.5 / n) ** 2
     for j in range(d):
         counts = np.bincount(bins[:, j].astype(np.int64), minlength=50)

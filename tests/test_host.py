@@ -77,6 +77,11 @@ def test_argument_errors_are_reported_without_device_calls(lib):
     fn.family = 42
     rc = lib.mv_vegas_fused(ctypes.byref(step), ctypes.byref(fn), None, None, None)
     assert rc == _abi.ERR_UNSUPPORTED and "family" in _abi.last_error()
+    fz = _abi.Finalize()
+    fz.n_dim, fz.n_out, fz.n_events = 4, 1, 100.0
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: the call is rejected first
+    rc = lib.mv_vegas_finalize(ctypes.byref(fz), fake, 202, _abi.ROWS + 1, None, fake, None, None)
+    assert rc == _abi.ERR_ARG and "n_rows" in _abi.last_error()
     with pytest.raises(ValueError):
         _abi.check(_abi.ERR_ARG)
     with pytest.raises(NotImplementedError):
@@ -275,3 +280,18 @@ def test_generate_condition_function():
     for args in ((1, "and"), (5, "bad"), (5, ["or", "and"]), (3, ["or", "bad"])):
         with pytest.raises(ValueError):
             generate_condition_function(*args)
+
+
+def test_chi2_per_dof_closed_form():
+    from vegasflow_amd.monte_carlo import chi2_per_dof
+
+    res = [(1.0, 0.1), (1.2, 0.2), (0.9, 0.1)]
+    w = np.array([100.0, 25.0, 100.0])
+    r = np.array([1.0, 1.2, 0.9])
+    avg = np.sum(w * r) / np.sum(w)
+    want = np.sum(w * (r - avg) ** 2) / 2
+    assert abs(chi2_per_dof(res, avg) - want) < 1e-14
+    assert np.isnan(chi2_per_dof(res[:1], 1.0))
+    vec = [(np.array([1.0, 2.0]), np.array([0.1, 0.1])), (np.array([1.1, 2.0]), np.array([0.1, 0.2]))]
+    out = chi2_per_dof(vec, np.array([1.05, 2.0]))
+    np.testing.assert_allclose(out, [0.5, 0.0])

@@ -252,3 +252,80 @@ def test_redistribution_closed_form():
     expect = np.maximum(2, damped * 1000 / 2 / damped.sum()).astype(np.int32)
     np.testing.assert_array_equal(got, expect)
     assert got.sum() <= 4 * 2 + 500
+
+
+def test_torch_cpu_baseline_matches_the_oracle_on_the_same_draws():
+    """The timed CPU baseline (torch op-for-op restatement with the dense
+    one-hot histogram) computes what the NumPy oracle computes."""
+    import torch
+
+    from oracle import torch_cpu as tc
+
+    d, n = 5, 3000
+    rng = np.random.default_rng(4)
+    grid = ora.initial_grid(d)
+    grid = ora.refine_grid(rng.random((d, 50)) + 0.1, grid)  # a non-uniform grid
+    u = ora.uniform_f64(3, 0, n, d)
+    res, res2, arr = tc.run_step(torch.from_numpy(np.ascontiguousarray(grid)), n, n, tc.gaussian, None,
+                                 rnds=torch.from_numpy(u))
+    x, w, ind = ora.vegas_map(u, grid)
+    x, w = ora.apply_xjac_and_limits(x, w, 1.0 / n)
+    ores, ores2, oarr = ora.vegas_event(x, w, ind, ora.gaussian(x, 0.1, 0.5))
+    assert abs(float(res) - ores) <= 1e-12 * abs(ores)
+    assert abs(float(res2) - ores2) <= 1e-12 * abs(ores2)
+    np.testing.assert_allclose(arr.numpy(), oarr, rtol=1e-12, atol=1e-300)
+
+
+def test_mixed_position_exact_and_tracks_the_f64_draws():
+    """The mixed path's (bin, frac) is xn = 50 (1 - (s + 1/2) / 2^23) exactly
+    (rational brute force), never on a bin edge, and lands in the bin of the
+    FP64 draw of the same word but for points within ~2^-23 of an edge."""
+    from fractions import Fraction
+
+    words = ora.hi_words(9, 2, 4000, 3).reshape(-1)
+    extreme = np.array([0, 1, 511, 512, 0x7FFFFFFF, 0xFFFFFDFF, 0xFFFFFFFF], dtype=np.uint32)
+    words = np.concatenate([words, extreme])
+    ind, frac = ora.mixed_position(words)
+    for k in list(range(0, words.size, 37)) + list(range(words.size - extreme.size, words.size)):
+        s = int(words[k]) >> 9
+        xn = 50 * (1 - Fraction(2 * s + 1, 2 ** 24))
+        assert ind[k] == int(xn) and Fraction(float(frac[k])) == xn - int(xn)
+    assert ind.min() >= 0 and ind.max() <= 49
+    assert frac.min() > 0 and frac.max() < 1
+    n, d = 20000, 4
+    _, _, ind64 = ora.vegas_map(ora.uniform_f64(5, 1, n, d), ora.initial_grid(d))
+    ind32, _ = ora.mixed_position(ora.hi_words(5, 1, n, d))
+    assert np.mean(ind32 != ind64) < 1e-3
+
+
+def _cuts_by_prefix_counts(t_res_sq, div):
+    """The rebin as the GPU computes it: cut k independently from the prefix
+    sums of the weights (count of prefix sums below k * ave)."""
+    r = np.asarray(t_res_sq, dtype=np.float64)
+    pad = np.concatenate([[0.0], r, [0.0]])
+    s = np.maximum(((pad[1:-1] + pad[2:]) + pad[:-2]) / np.array([2.0] + [3.0] * 48 + [2.0]), 1e-30)
+    S = np.sum(s)
+    wei = np.power((1.0 - s / S) / (np.log(S) - np.log(s)), 1.5)
+    ave = np.sum(wei) / 50
+    c = np.cumsum(wei)
+    out = [0.0]
+    for k in range(1, 50):
+        t = k * ave
+        n = min(int(np.sum(c < t)), 49)
+        cur, prev = div[n + 1], (0.0 if n == 0 else div[n])
+        out.append(cur - (cur - prev) * (c[n] - t) / wei[n])
+    return np.array(out + [1.0])
+
+
+def test_parallel_cut_form_equals_serial_rebin():
+    """Cut k of the serial 49-cut loop (vflow.py:194-207) stops in the first
+    bin whose running weight reaches k * ave: the per-cut form agrees with
+    the serial oracle to rounding on varied histograms."""
+    rng = np.random.default_rng(8)
+    cases = [np.ones(50), np.zeros(50), np.eye(50)[17] * 3.0, np.eye(50)[0], np.eye(50)[49],
+             rng.gamma(0.3, size=50), rng.random(50) ** 8, np.exp(-np.linspace(-4, 4, 50) ** 2 * 3)]
+    for k, h in enumerate(cases):
+        div = ora.initial_grid(1)[0] if k % 2 == 0 else np.sort(np.concatenate([[0.0, 1.0], rng.random(49)]))
+        a = ora.refine_grid_per_dimension(h, div)
+        b = _cuts_by_prefix_counts(h, div)
+        np.testing.assert_allclose(b, a, rtol=1e-13, atol=1e-16)

@@ -55,8 +55,8 @@ class BatchVegas:
             fn, d, n, seed = spec
             if not isinstance(fn, BuiltinIntegrand) or not fn.fused_ok(int(d)) or int(d) > _abi.FUSED_MAX_DIM:
                 raise ValueError(f"batch items need a built-in integrand with n_dim <= {_abi.FUSED_MAX_DIM}")
-            if int(n) < 2:
-                raise ValueError("every item needs at least 2 events per iteration")
+            if int(n) < 2 or int(n) >= 1 << 32:
+                raise ValueError("every item needs 2 .. 2^32 - 1 events per iteration")
             self.specs.append((fn, int(d), int(n), int(seed)))
         self.n_items = len(self.specs)
         if self.n_items == 0:
@@ -222,6 +222,17 @@ class BatchVegas:
         k = int(self.draws.max().item())
         h = self.history_buf.view(self.n_items, self.max_iter, 2).cpu().numpy()
         return h[:, : min(k, self.max_iter)]
+
+    def chi2_dof(self):
+        """Per-item chi^2 per degree of freedom of the recorded iterations
+        around their weighted average (FP64, host; NaN below 2 iterations)."""
+        h = self.history()
+        if h.shape[1] < 2:
+            return np.full(self.n_items, np.nan)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            w = 1.0 / np.square(h[:, :, 1])
+            avg = np.sum(w * h[:, :, 0], axis=1) / np.sum(w, axis=1)
+            return np.sum(w * np.square(h[:, :, 0] - avg[:, None]), axis=1) / (h.shape[1] - 1)
 
     def iterations(self):
         return self.draws.cpu().numpy()

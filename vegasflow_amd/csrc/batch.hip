@@ -32,15 +32,18 @@ __device__ __forceinline__ void batch_events(const mv_batch_item& it, const mv_b
   const double wscale = 1.0 / (double)it.n_events;  // xjac, monte_carlo.py:224-227
   double* wh = hist + (threadIdx.x >> 6) * nb;
   double acc = 0.0, acc2 = 0.0;
+  // a tile never crosses a multiple of 2^32 events (mv_batch_fused), so the
+  // high counter word is uniform over the tile
+  const uint32_t lo0 = (uint32_t)((uint64_t)tile.first & 0xffffffffu);
+  const uint32_t hi = (uint32_t)((uint64_t)tile.first >> 32);
   for (int k = threadIdx.x; k < tile.count; k += kBlock) {
-    const uint64_t ge = (uint64_t)(tile.first + k);
     double tmp;
     if constexpr (MIXED)
-      tmp = event_mixed<FAM, TRAIN>(F, (uint32_t)ge, (uint32_t)(ge >> 32), draw, key, groups, d,
+      tmp = event_mixed<FAM, TRAIN>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
                                     static_cast<const float2*>(tab), static_cast<const float*>(tw), wh,
                                     nullptr, wscale);
     else
-      tmp = event_f64<FAM, TRAIN>(F, (uint32_t)ge, (uint32_t)(ge >> 32), draw, key, groups, d,
+      tmp = event_f64<FAM, TRAIN>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
                                   static_cast<const double2*>(tab), static_cast<const double*>(tw), wh,
                                   nullptr, wscale);
     acc += tmp;

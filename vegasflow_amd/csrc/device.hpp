@@ -64,6 +64,20 @@ __device__ __forceinline__ double draw_f64(uint32_t hi, uint32_t lo) {
   return r * kSpan + kTechCut;
 }
 
+// Mixed-precision position of a draw in one dimension, from the 32-bit word
+// hw alone. The leading 23 bits s = hw >> 9 stand for 1 - u, so the point
+// tracks the FP64 draw of the same word to 2^-23: xn = 50 (1 - (s + 1/2) /
+// 2^23), held exactly in fixed point as p = xn 2^23 = 50 2^23 - 25 - 50 s
+// (a 24x24-bit multiply-add; p is odd, so xn never sits on a bin edge).
+// Returns the bin p >> 23 (always 0..49) and sets one_plus_frac = 1 + frac,
+// the float whose mantissa is the 23 fraction bits of p.
+__device__ __forceinline__ int mixed_position(uint32_t hw, float& one_plus_frac) {
+  const uint32_t s = hw >> 9;
+  const uint32_t p = 419430375u - 50u * s;  // 50 * 2^23 - 25 - 50 s
+  one_plus_frac = __uint_as_float((p & 0x7FFFFFu) | 0x3F800000u);
+  return (int)(p >> 23);
+}
+
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------

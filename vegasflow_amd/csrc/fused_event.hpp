@@ -3,7 +3,8 @@
 //
 // Per (dimension, bin) the LDS table holds the map frac -> y = P + Q frac of
 // the integrand coordinate (Integrand::fold, integration limits folded in)
-// and the Jacobian factor W = dx * 50 (vflow.py:73, 78).
+// and the Jacobian factor W = dx * 50 (vflow.py:73, 78). The FP32 table of
+// the mixed path takes 1 + frac (mixed_position) and so stores (P - Q, Q).
 #pragma once
 
 #include <type_traits>
@@ -30,7 +31,7 @@ __device__ __forceinline__ void build_table(const mv_integrand& fn, const double
       Q = dx * xdelta[j];
     }
     Integrand<T, FAM>::fold(fn, j, P, Q);
-    tab[i].x = (T)P;
+    tab[i].x = sizeof(T) == 4 ? (T)(P - Q) : (T)P;
     tab[i].y = (T)Q;
     tw[i] = (T)(dx * kFBins);  // vflow.py:78
   }
@@ -87,10 +88,8 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
   return tmp;
 }
 
-// One event, mixed precision: FP32 map and integrand from the 32 high bits
-// of each draw, FP64 result. Bin and fraction come from the 32x32->64
-// product s * 50 with s = ~hi, i.e. xn = 50 (1 - u) in fixed point, which
-// keeps the bin in [0, 49].
+// One event, mixed precision: FP32 map and integrand from the high word of
+// each draw (mixed_position), FP64 result.
 template <int FAM, bool TRAIN>
 __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
                                               uint32_t draw, Key key, int groups, int d,
@@ -108,12 +107,10 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
       for (int q = 0; q < 4; ++q) {
         const int j = 4 * g + q;
         if (j < d) {
-          const uint32_t hw = inj ? inj[j] : word(h, q);
-          const uint64_t p = (uint64_t)(~hw) * (uint64_t)kBins;
-          const int ind = (int)(p >> 32);
-          const float frac = (float)(((uint32_t)p) >> 8) * 0x1p-24f;
+          float f1;
+          const int ind = mixed_position(inj ? inj[j] : word(h, q), f1);
           const float2 t = tab[j * kBins + ind];
-          F.add_folded(j, fmaf(t.y, frac, t.x));
+          F.add_folded(j, fmaf(t.y, f1, t.x));
           w = w * tw[j * kBins + ind];
           if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
         }

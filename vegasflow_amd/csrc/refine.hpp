@@ -7,10 +7,14 @@
 namespace mv {
 
 // refine_grid_per_dimension (vflow.py:129-211) on one wave.
-// Lanes 0..49 own a bin, lanes 0..50 an edge; the 49-cut rebin loop runs on
-// wave-uniform values with readlane broadcasts, in the reference's order.
-// One addition: the inner loop stops at the last bin (n <= 49); the
-// reference has no guard there.
+// Lanes 0..49 own a bin, lanes 0..50 an edge. The reference's 49 serial cuts
+// (vflow.py:194-207) are computed independently, lane k making cut k: the
+// serial loop stops in the first bin n whose running weight reaches k * ave,
+// with bw = cumsum(wei)[n] - k * ave left over, so lane k finds n by
+// counting the prefix sums below k * ave and interpolates as the reference
+// does, prev = 0.0 (not div[0]) for n = 0 included. Only the rounding of bw
+// differs from the serial accumulation (a few ulp of sum(wei)). n is capped
+// at the last bin; the reference has no guard there.
 __device__ inline void refine_wave(const double* h /* LDS, 50 */, double* __restrict__ div /* 51 */) {
   const int lane = threadIdx.x & 63;
   const double edge = lane <= kBins ? div[lane] : 0.0;
@@ -31,21 +35,28 @@ __device__ inline void refine_wave(const double* h /* LDS, 50 */, double* __rest
   }
   const double ave = bcast_lane(wave_sum(wei), 0) / (double)kBins;  // vflow.py:166
 
-  double mine = lane == 0 ? 0.0 : 1.0;  // edges 0 and 50 (vflow.py:195, 208)
-  double bw = 0.0, cur = 0.0, prev = 0.0;
-  int n = -1;
-  for (int k = 1; k < kBins; ++k) {  // vflow.py:201-207
-    while (bw < ave && n < kBins - 1) {
-      ++n;
-      bw += bcast_lane(wei, n);
-      prev = cur;
-      cur = bcast_lane(edge, n + 1);
-    }
-    bw -= ave;
-    // n < 0 only if the weights are not finite (NaN integrand values)
-    const double delta = (cur - prev) * bw / bcast_lane(wei, n < 0 ? 0 : n);
-    if (lane == k) mine = cur - delta;
+  // inclusive prefix sums of the weights (lanes >= 50 hold 0)
+  double c = wei;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double t = __shfl_up(c, o, 64);
+    if (lane >= o) c += t;
   }
+  const double target = (double)lane * ave;  // cut k = lane
+  int n = 0;
+  for (int b = 0; b < kBins; ++b) n += bcast_lane(c, b) < target ? 1 : 0;
+  if (n > kBins - 1) n = kBins - 1;
+  // every shuffle runs on the whole wave (a lane-gather from an inactive
+  // lane reads 0); the n == 0 case is selected afterwards
+  const double cn = __shfl(c, n, 64);
+  const double wn = __shfl(wei, n, 64);
+  const double en = __shfl(edge, n, 64);
+  const double cur = __shfl(edge, n + 1, 64);      // vflow.py:193
+  const double bw = cn - target;                   // vflow.py:205
+  const double prev = n == 0 ? 0.0 : en;           // vflow.py:197-200
+  const double delta = (cur - prev) * bw / wn;     // vflow.py:206
+  double mine = lane == 0 ? 0.0 : 1.0;  // edges 0 and 50 (vflow.py:195, 208)
+  if (lane >= 1 && lane < kBins) mine = cur - delta;             // vflow.py:207
   if (lane <= kBins) div[lane] = mine;
 }
 
@@ -54,15 +65,21 @@ struct FzK {
   double n_events;
 };
 
-// Sum column c of the group rows in order and zero it for the next producer.
+// Sum column c of the (at most MV_ROWS) group rows in order and zero it for
+// the next producer. All loads are issued before the first store so that
+// they are in flight together instead of one memory latency per row.
 __device__ __forceinline__ double take_column(double* __restrict__ P, int64_t stride, int nrows,
                                               int64_t c) {
-  double a = P[c];
-  P[c] = 0.0;
-  for (int r = 1; r < nrows; ++r) {
-    a += P[(int64_t)r * stride + c];
-    P[(int64_t)r * stride + c] = 0.0;
-  }
+  double v[MV_ROWS];
+#pragma unroll
+  for (int r = 0; r < MV_ROWS; ++r) v[r] = r < nrows ? P[(int64_t)r * stride + c] : 0.0;
+  double a = v[0];
+#pragma unroll
+  for (int r = 1; r < MV_ROWS; ++r)
+    if (r < nrows) a += v[r];
+#pragma unroll
+  for (int r = 0; r < MV_ROWS; ++r)
+    if (r < nrows) P[(int64_t)r * stride + c] = 0.0;
   return a;
 }
 

@@ -24,6 +24,10 @@ struct StepK {
   int groups;  // Philox blocks of high words: ceil(n_dim / 4)
   int64_t n_events;
   int64_t event_offset;
+  // event_offset split into the two low Philox counter words; for launches
+  // cut by for_each_counter_range, event i of the launch has counter words
+  // (ev_lo + i, ev_hi) with no carry, so the high word is launch-uniform
+  uint32_t ev_lo, ev_hi;
   Key key;
   uint32_t draw_id;
   double xjac;
@@ -39,6 +43,8 @@ inline StepK make_stepk(const mv_step& s) {
   k.groups = (s.n_dim + 3) / 4;
   k.n_events = s.n_events;
   k.event_offset = s.event_offset;
+  k.ev_lo = (uint32_t)((uint64_t)s.event_offset & 0xffffffffu);
+  k.ev_hi = (uint32_t)((uint64_t)s.event_offset >> 32);
   k.key.k0 = (uint32_t)(s.seed & 0xffffffffu);
   k.key.k1 = (uint32_t)(s.seed >> 32);
   k.draw_id = s.draw_id;
@@ -48,6 +54,31 @@ inline StepK make_stepk(const mv_step& s) {
   k.xdelta = s.xdelta;
   k.inject = s.inject;
   return k;
+}
+
+// Run ``launch(StepK, first)`` over pieces of the step's events that do not
+// cross a multiple of 2^32 in the event counter (a single piece unless the
+// range crosses one), advancing event_offset and the injected draws; ``first``
+// is the index of the piece's first event within the step (for outputs).
+// Returns the largest value ``launch`` returned (its workgroup count).
+template <typename F>
+int for_each_counter_range(const mv_step& step, size_t inject_elem_bytes, F&& launch) {
+  mv_step piece = step;
+  int most = 0;
+  int64_t left = step.n_events;
+  do {
+    const uint64_t lo = (uint64_t)piece.event_offset & 0xffffffffull;
+    const int64_t room = (int64_t)((1ull << 32) - lo);
+    piece.n_events = left < room ? left : room;
+    const int b = launch(make_stepk(piece), step.n_events - left);
+    most = b > most ? b : most;
+    left -= piece.n_events;
+    if (piece.inject)
+      piece.inject = static_cast<const char*>(piece.inject) +
+                     (size_t)piece.n_events * (size_t)piece.n_dim * inject_elem_bytes;
+    piece.event_offset += piece.n_events;
+  } while (left > 0);
+  return most;
 }
 
 struct RowsK {
@@ -63,6 +94,25 @@ inline int producer_blocks(int64_t n_events) {
 }
 
 inline int rows_used(int blocks) { return blocks < kRows ? blocks : kRows; }
+
+// Workgroups of ``kernel`` (kBlock threads, ``lds`` dynamic LDS bytes) that
+// are resident on the current device at once: occupancy per CU x CUs, cached
+// per (device, kernel, lds). 0 if the runtime cannot tell.
+int resident_blocks(const void* kernel, size_t lds);
+
+// Launch a grid-stride producer over ``n`` events with producer_blocks(n)
+// workgroups capped at one device-wide round of resident workgroups: every
+// workgroup owns an equal share of the events, so a grid larger than what
+// fits at once would leave a second, mostly idle round at the end.
+// Returns the number of workgroups launched.
+template <typename... KArgs, typename... Args>
+int launch_producer(void (*kernel)(KArgs...), int64_t n, size_t lds, hipStream_t st, Args... args) {
+  int blocks = producer_blocks(n);
+  const int fit = resident_blocks(reinterpret_cast<const void*>(kernel), lds);
+  if (fit > 0 && blocks > fit) blocks = fit;
+  kernel<<<dim3(blocks), dim3(kBlock), lds, st>>>(args...);
+  return blocks;
+}
 
 inline int make_rows(const mv_partials* p, int64_t need_stride, const char* what, RowsK* out) {
   if (!p || !p->rows) return fail(MV_ERR_ARG, "%s: partials workspace missing", what);

@@ -10,6 +10,9 @@
 //       per-dimension refinement (one workgroup per dimension).
 //   plain_*   PlainFlow variants.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "abi_util.hpp"
@@ -23,6 +26,29 @@ namespace mv {
 std::string& last_error_slot() {
   static thread_local std::string s;
   return s;
+}
+
+int resident_blocks(const void* kernel, size_t lds) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void*, size_t>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  const auto key = std::make_tuple(dev, kernel, lds);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  const int r = cus * per_cu;
+  std::lock_guard<std::mutex> g(mu);
+  cache[key] = r;
+  return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -50,8 +76,7 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand 
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint64_t ge = (uint64_t)(s.event_offset + i);
-    const double tmp = event_f64<FAM, TRAIN>(F, (uint32_t)ge, (uint32_t)(ge >> 32), s.draw_id, s.key,
+    const double tmp = event_f64<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id, s.key,
                                              s.groups, d, tab, tw, wh, inj ? inj + i * d : nullptr,
                                              wscale);
     acc += tmp;
@@ -81,8 +106,7 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_mixed(StepK s, mv_integran
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint64_t ge = (uint64_t)(s.event_offset + i);
-    const double tmp = event_mixed<FAM, TRAIN>(F, (uint32_t)ge, (uint32_t)(ge >> 32), s.draw_id, s.key,
+    const double tmp = event_mixed<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id, s.key,
                                                s.groups, d, tab, tw, wh, inj ? inj + i * d : nullptr,
                                                wscale);
     acc += tmp;
@@ -103,8 +127,7 @@ __global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, 
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint64_t ge = (uint64_t)(s.event_offset + i);
-    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
     F.begin();
     for (int g = 0; g < s.groups; ++g) {
       uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
@@ -152,8 +175,7 @@ __global__ __launch_bounds__(kBlock) void vegas_sample_f64(StepK s, const double
   const double* inj = static_cast<const double*>(s.inject);
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint64_t ge = (uint64_t)(s.event_offset + i);
-    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
     double w = 1.0;
     for (int g = 0; g < s.groups; ++g) {
       uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
@@ -195,8 +217,7 @@ __global__ __launch_bounds__(kBlock) void vegas_sample_mixed(StepK s, const doub
   const float wscale = (float)(limits ? s.xjac * s.xdeltajac : s.xjac);
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint64_t ge = (uint64_t)(s.event_offset + i);
-    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
     float w = 1.0f;
     for (int g = 0; g < s.groups; ++g) {
       uint4 h = make_uint4(0, 0, 0, 0);
@@ -205,10 +226,9 @@ __global__ __launch_bounds__(kBlock) void vegas_sample_mixed(StepK s, const doub
       for (int q = 0; q < 4; ++q) {
         const int j = 4 * g + q;
         if (j < d) {
-          const uint32_t hw = inj ? inj[i * d + j] : word(h, q);
-          const uint64_t p = (uint64_t)(~hw) * (uint64_t)kBins;
-          const int ind = (int)(p >> 32);
-          const float frac = (float)(((uint32_t)p) >> 8) * 0x1p-24f;
+          float f1;
+          const int ind = mixed_position(inj ? inj[i * d + j] : word(h, q), f1);
+          const float frac = f1 - 1.0f;  // exact
           const double lo = grid[j * (kBins + 1) + ind];
           const double dx = grid[j * (kBins + 1) + ind + 1] - lo;
           float x = fmaf((float)dx, frac, (float)lo);
@@ -229,8 +249,7 @@ __global__ __launch_bounds__(kBlock) void plain_sample(StepK s, void* __restrict
   const bool limits = s.xmin != nullptr;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint64_t ge = (uint64_t)(s.event_offset + i);
-    const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
+    const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
     for (int g = 0; g < s.groups; ++g) {
       uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
       if (!s.inject) {
@@ -440,31 +459,25 @@ extern "C" int32_t mv_producer_rows(int64_t n_events) {
 namespace {
 
 template <int FAM>
-void launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool train,
-                        const double* grid, const RowsK& R, int blocks, hipStream_t st) {
+int launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool train,
+                       const double* grid, const RowsK& R, hipStream_t st) {
   const int nb = k.n_dim * kBins;
+  const int64_t n = k.n_events;
   if (!mixed) {
     const size_t lds = (size_t)(3 * nb + (train ? kWaves * nb : 0)) * sizeof(double);
-    if (train)
-      hipLaunchKernelGGL((vegas_fused_f64<FAM, true>), dim3(blocks), dim3(kBlock), lds, st, k, fn, grid, R);
-    else
-      hipLaunchKernelGGL((vegas_fused_f64<FAM, false>), dim3(blocks), dim3(kBlock), lds, st, k, fn, grid, R);
-  } else {
-    const size_t lds = (size_t)(nb + (nb + 1) / 2 + (train ? kWaves * nb : 0)) * sizeof(double);
-    if (train)
-      hipLaunchKernelGGL((vegas_fused_mixed<FAM, true>), dim3(blocks), dim3(kBlock), lds, st, k, fn, grid, R);
-    else
-      hipLaunchKernelGGL((vegas_fused_mixed<FAM, false>), dim3(blocks), dim3(kBlock), lds, st, k, fn, grid, R);
+    return train ? launch_producer(vegas_fused_f64<FAM, true>, n, lds, st, k, fn, grid, R)
+                 : launch_producer(vegas_fused_f64<FAM, false>, n, lds, st, k, fn, grid, R);
   }
+  const size_t lds = (size_t)(nb + (nb + 1) / 2 + (train ? kWaves * nb : 0)) * sizeof(double);
+  return train ? launch_producer(vegas_fused_mixed<FAM, true>, n, lds, st, k, fn, grid, R)
+               : launch_producer(vegas_fused_mixed<FAM, false>, n, lds, st, k, fn, grid, R);
 }
 
 template <int FAM>
-void launch_plain_fused(const StepK& k, const mv_integrand& fn, bool mixed, const RowsK& R,
-                        int blocks, hipStream_t st) {
-  if (mixed)
-    hipLaunchKernelGGL((plain_fused<FAM, true>), dim3(blocks), dim3(kBlock), 0, st, k, fn, R);
-  else
-    hipLaunchKernelGGL((plain_fused<FAM, false>), dim3(blocks), dim3(kBlock), 0, st, k, fn, R);
+int launch_plain_fused(const StepK& k, const mv_integrand& fn, bool mixed, const RowsK& R,
+                       hipStream_t st) {
+  return mixed ? launch_producer(plain_fused<FAM, true>, k.n_events, 0, st, k, fn, R)
+               : launch_producer(plain_fused<FAM, false>, k.n_events, 0, st, k, fn, R);
 }
 
 int sample_blocks(int64_t n) {
@@ -486,17 +499,17 @@ extern "C" int mv_vegas_fused(const mv_step* step, const mv_integrand* fn, const
   if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train),
                       "mv_vegas_fused", &R)))
     return rc;
-  const int blocks = producer_blocks(step->n_events);
-  const StepK k = make_stepk(*step);
   const bool mixed = step->precision == MV_MIXED, train = step->train != 0;
   hipStream_t st = as_stream(stream);
-  switch (fn->family) {
-    case MV_FN_CONSTANT: launch_vegas_fused<MV_FN_CONSTANT>(k, *fn, mixed, train, grid, R, blocks, st); break;
-    case MV_FN_GAUSSIAN: launch_vegas_fused<MV_FN_GAUSSIAN>(k, *fn, mixed, train, grid, R, blocks, st); break;
-    case MV_FN_PRODUCT: launch_vegas_fused<MV_FN_PRODUCT>(k, *fn, mixed, train, grid, R, blocks, st); break;
-    case MV_FN_GAUSSMIX: launch_vegas_fused<MV_FN_GAUSSMIX>(k, *fn, mixed, train, grid, R, blocks, st); break;
-    case MV_FN_BWPOWER: launch_vegas_fused<MV_FN_BWPOWER>(k, *fn, mixed, train, grid, R, blocks, st); break;
-  }
+  const int blocks = for_each_counter_range(*step, mixed ? 4 : 8, [&](const StepK& k, int64_t) {
+    switch (fn->family) {
+      case MV_FN_CONSTANT: return launch_vegas_fused<MV_FN_CONSTANT>(k, *fn, mixed, train, grid, R, st);
+      case MV_FN_GAUSSIAN: return launch_vegas_fused<MV_FN_GAUSSIAN>(k, *fn, mixed, train, grid, R, st);
+      case MV_FN_PRODUCT: return launch_vegas_fused<MV_FN_PRODUCT>(k, *fn, mixed, train, grid, R, st);
+      case MV_FN_GAUSSMIX: return launch_vegas_fused<MV_FN_GAUSSMIX>(k, *fn, mixed, train, grid, R, st);
+      default: return launch_vegas_fused<MV_FN_BWPOWER>(k, *fn, mixed, train, grid, R, st);
+    }
+  });
   if ((rc = check_launch("mv_vegas_fused"))) return rc;
   return rows_used(blocks);
 }
@@ -514,17 +527,17 @@ extern "C" int mv_plain_fused(const mv_step* step, const mv_integrand* fn, const
   if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, 0),
                       "mv_plain_fused", &R)))
     return rc;
-  const int blocks = producer_blocks(step->n_events);
-  const StepK k = make_stepk(*step);
   const bool mixed = step->precision == MV_MIXED;
   hipStream_t st = as_stream(stream);
-  switch (fn->family) {
-    case MV_FN_CONSTANT: launch_plain_fused<MV_FN_CONSTANT>(k, *fn, mixed, R, blocks, st); break;
-    case MV_FN_GAUSSIAN: launch_plain_fused<MV_FN_GAUSSIAN>(k, *fn, mixed, R, blocks, st); break;
-    case MV_FN_PRODUCT: launch_plain_fused<MV_FN_PRODUCT>(k, *fn, mixed, R, blocks, st); break;
-    case MV_FN_GAUSSMIX: launch_plain_fused<MV_FN_GAUSSMIX>(k, *fn, mixed, R, blocks, st); break;
-    case MV_FN_BWPOWER: launch_plain_fused<MV_FN_BWPOWER>(k, *fn, mixed, R, blocks, st); break;
-  }
+  const int blocks = for_each_counter_range(*step, mixed ? 4 : 8, [&](const StepK& k, int64_t) {
+    switch (fn->family) {
+      case MV_FN_CONSTANT: return launch_plain_fused<MV_FN_CONSTANT>(k, *fn, mixed, R, st);
+      case MV_FN_GAUSSIAN: return launch_plain_fused<MV_FN_GAUSSIAN>(k, *fn, mixed, R, st);
+      case MV_FN_PRODUCT: return launch_plain_fused<MV_FN_PRODUCT>(k, *fn, mixed, R, st);
+      case MV_FN_GAUSSMIX: return launch_plain_fused<MV_FN_GAUSSMIX>(k, *fn, mixed, R, st);
+      default: return launch_plain_fused<MV_FN_BWPOWER>(k, *fn, mixed, R, st);
+    }
+  });
   if ((rc = check_launch("mv_plain_fused"))) return rc;
   return rows_used(blocks);
 }
@@ -535,15 +548,20 @@ extern "C" int mv_vegas_sample(const mv_step* step, const double* grid, void* x,
   if (rc) return rc;
   if (!grid || !x || !w) return fail(MV_ERR_ARG, "mv_vegas_sample: grid, x and w are required");
   if (step->n_events == 0) return MV_OK;
-  const StepK k = make_stepk(*step);
   hipStream_t st = as_stream(stream);
-  const int blocks = sample_blocks(step->n_events);
-  if (step->precision == MV_F64)
-    hipLaunchKernelGGL(vegas_sample_f64, dim3(blocks), dim3(kBlock), 0, st, k, grid,
-                       static_cast<double*>(x), static_cast<double*>(w), idx);
-  else
-    hipLaunchKernelGGL(vegas_sample_mixed, dim3(blocks), dim3(kBlock), 0, st, k, grid,
-                       static_cast<float*>(x), static_cast<float*>(w), idx);
+  const bool f64 = step->precision == MV_F64;
+  const int64_t d = step->n_dim;
+  for_each_counter_range(*step, f64 ? 8 : 4, [&](const StepK& k, int64_t e0) {
+    const int blocks = sample_blocks(k.n_events);
+    uint8_t* io = idx ? idx + e0 * d : nullptr;
+    if (f64)
+      hipLaunchKernelGGL(vegas_sample_f64, dim3(blocks), dim3(kBlock), 0, st, k, grid,
+                         static_cast<double*>(x) + e0 * d, static_cast<double*>(w) + e0, io);
+    else
+      hipLaunchKernelGGL(vegas_sample_mixed, dim3(blocks), dim3(kBlock), 0, st, k, grid,
+                         static_cast<float*>(x) + e0 * d, static_cast<float*>(w) + e0, io);
+    return blocks;
+  });
   return check_launch("mv_vegas_sample");
 }
 
@@ -552,13 +570,19 @@ extern "C" int mv_plain_sample(const mv_step* step, void* x, void* stream) {
   if (rc) return rc;
   if (!x) return fail(MV_ERR_ARG, "mv_plain_sample: x is required");
   if (step->n_events == 0) return MV_OK;
-  const StepK k = make_stepk(*step);
   hipStream_t st = as_stream(stream);
-  const int blocks = sample_blocks(step->n_events);
-  if (step->precision == MV_F64)
-    hipLaunchKernelGGL(plain_sample<false>, dim3(blocks), dim3(kBlock), 0, st, k, x);
-  else
-    hipLaunchKernelGGL(plain_sample<true>, dim3(blocks), dim3(kBlock), 0, st, k, x);
+  const bool f64 = step->precision == MV_F64;
+  const int64_t d = step->n_dim;
+  for_each_counter_range(*step, f64 ? 8 : 4, [&](const StepK& k, int64_t e0) {
+    const int blocks = sample_blocks(k.n_events);
+    if (f64)
+      hipLaunchKernelGGL(plain_sample<false>, dim3(blocks), dim3(kBlock), 0, st, k,
+                         static_cast<void*>(static_cast<double*>(x) + e0 * d));
+    else
+      hipLaunchKernelGGL(plain_sample<true>, dim3(blocks), dim3(kBlock), 0, st, k,
+                         static_cast<void*>(static_cast<float*>(x) + e0 * d));
+    return blocks;
+  });
   return check_launch("mv_plain_sample");
 }
 
@@ -608,7 +632,8 @@ extern "C" int mv_vegas_finalize(const mv_finalize* fz, double* rows, int64_t ro
   if (!fz) return fail(MV_ERR_ARG, "mv_vegas_finalize: fz is NULL");
   if (fz->n_dim < 1 || fz->n_out < 1 || fz->n_out > MV_MAX_OUT)
     return fail(MV_ERR_ARG, "mv_vegas_finalize: bad n_dim/n_out");
-  if (!rows || !out || n_rows < 1) return fail(MV_ERR_ARG, "mv_vegas_finalize: rows/out/n_rows");
+  if (!rows || !out || n_rows < 1 || n_rows > MV_ROWS)
+    return fail(MV_ERR_ARG, "mv_vegas_finalize: rows/out missing or n_rows %d not in 1..%d", n_rows, MV_ROWS);
   if (row_stride < mv_partial_stride(fz->n_dim, fz->n_out, fz->train))
     return fail(MV_ERR_ARG, "mv_vegas_finalize: row_stride too small");
   if (fz->refine && (!fz->train || !grid))

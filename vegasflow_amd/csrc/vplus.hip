@@ -439,16 +439,16 @@ int validate_strata(const mv_strata* st, int n_dim, const char* what) {
 StrataK make_strata(const mv_strata& s) { return StrataK{s.n_strat, s.n_cubes, s.n_ev, s.offsets}; }
 
 template <int FAM>
-void launch_vplus_fused(const StepK& k, const StrataK& st, const mv_integrand& fn, bool mixed,
-                        bool train, const double* grid, double* ress, double* ress2, const RowsK& R,
-                        int blocks, hipStream_t s) {
+int launch_vplus_fused(const StepK& k, const StrataK& st, const mv_integrand& fn, bool mixed,
+                       bool train, const double* grid, double* ress, double* ress2, const RowsK& R,
+                       hipStream_t s) {
   const int nb = k.n_dim * kBins;
   const size_t hist = train ? (size_t)kWaves * nb : 0;
   const size_t lds = ((mixed ? nb + (nb + 1) / 2 : 3 * nb) + hist) * sizeof(double);
 #define MV_VPF(TR, MX) \
-  hipLaunchKernelGGL((vplus_fused<FAM, TR, MX>), dim3(blocks), dim3(kBlock), lds, s, k, st, fn, grid, ress, ress2, R)
-  if (mixed) { if (train) MV_VPF(true, true); else MV_VPF(false, true); }
-  else { if (train) MV_VPF(true, false); else MV_VPF(false, false); }
+  launch_producer(vplus_fused<FAM, TR, MX>, k.n_events, lds, s, k, st, fn, grid, ress, ress2, R)
+  if (mixed) return train ? MV_VPF(true, true) : MV_VPF(false, true);
+  return train ? MV_VPF(true, false) : MV_VPF(false, false);
 #undef MV_VPF
 }
 
@@ -467,17 +467,17 @@ extern "C" int mv_vplus_fused(const mv_step* step, const mv_strata* strata, cons
   if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, 1, step->train),
                       "mv_vplus_fused", &R)))
     return rc;
-  const int blocks = producer_blocks(step->n_events);
   const StepK k = make_stepk(*step);
   const StrataK st = make_strata(*strata);
   const bool mixed = step->precision == MV_MIXED, train = step->train != 0;
+  int blocks = 1;
   hipStream_t s = as_stream(stream);
   switch (fn->family) {
-    case MV_FN_CONSTANT: launch_vplus_fused<MV_FN_CONSTANT>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
-    case MV_FN_GAUSSIAN: launch_vplus_fused<MV_FN_GAUSSIAN>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
-    case MV_FN_PRODUCT: launch_vplus_fused<MV_FN_PRODUCT>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
-    case MV_FN_GAUSSMIX: launch_vplus_fused<MV_FN_GAUSSMIX>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
-    case MV_FN_BWPOWER: launch_vplus_fused<MV_FN_BWPOWER>(k, st, *fn, mixed, train, grid, ress, ress2, R, blocks, s); break;
+    case MV_FN_CONSTANT: blocks = launch_vplus_fused<MV_FN_CONSTANT>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
+    case MV_FN_GAUSSIAN: blocks = launch_vplus_fused<MV_FN_GAUSSIAN>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
+    case MV_FN_PRODUCT: blocks = launch_vplus_fused<MV_FN_PRODUCT>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
+    case MV_FN_GAUSSMIX: blocks = launch_vplus_fused<MV_FN_GAUSSMIX>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
+    case MV_FN_BWPOWER: blocks = launch_vplus_fused<MV_FN_BWPOWER>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
   }
   if ((rc = check_launch("mv_vplus_fused"))) return rc;
   return rows_used(blocks);
@@ -545,7 +545,9 @@ extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t
                      offsets, ress, ress2, adaptive, min_neval, init_calls, beta, out, arr_var_out);
   if ((rc = check_launch("mv_vplus_finalize"))) return rc;
   if (train) {  // histogram columns are always consumed
-    if (!rows || n_rows < 1) return fail(MV_ERR_ARG, "mv_vplus_finalize: histogram rows required");
+    if (!rows || n_rows < 1 || n_rows > MV_ROWS)
+      return fail(MV_ERR_ARG, "mv_vplus_finalize: histogram rows missing or n_rows %d not in 1..%d", n_rows,
+                  MV_ROWS);
     if (row_stride < mv_partial_stride(n_dim, 1, 1))
       return fail(MV_ERR_ARG, "mv_vplus_finalize: row_stride too small");
     if (refine && !grid) return fail(MV_ERR_ARG, "mv_vplus_finalize: grid required to refine");

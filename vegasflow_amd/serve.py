@@ -96,8 +96,8 @@ class Server:
         if not isinstance(req.integrand, BuiltinIntegrand) or not req.integrand.fused_ok(req.n_dim) \
                 or req.n_dim > MAX_DIM:
             raise ValueError(f"requests need a built-in integrand with n_dim <= {MAX_DIM}")
-        if req.n_events < 2:
-            raise ValueError("a request needs at least 2 events per iteration")
+        if req.n_events < 2 or req.n_events >= 1 << 32:
+            raise ValueError("a request needs 2 .. 2^32 - 1 events per iteration")
         self.queue.append(req)
 
     def _admit(self, req, now):
