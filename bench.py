@@ -30,6 +30,12 @@ N = 10_000_000
 A = 0.1
 SEED = 11
 PEAK = {"mixed": 157.3, "f64": 78.6}  # TFLOP/s, MI355X datasheet (FP32 / FP64 vector)
+# HBM bytes per launch of the fused training kernels at this workload, from the
+# FETCH_SIZE and WRITE_SIZE passes of rocprofv3 scaled by a 1 GiB copy
+# (profiles/vegas_fused_gauss_d8_1e7_pmc.md): 0 bytes per event; the grid
+# reads and the FP64 atomic adds of the partial rows
+TRAFFIC = {"mixed": 6.146e6, "f64": 5.314e6}
+TRAFFIC_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_pmc.md"
 
 
 def flops_per_event(d, train=True):
@@ -380,11 +386,15 @@ def main():
             "peak": PEAK["mixed"],
             "unit": "TFLOP/s",
             "frac": achieved / PEAK["mixed"],
-            "traffic": None,
+            "traffic": TRAFFIC["mixed"],
+            "traffic_unit": "bytes per launch",
+            "traffic_source": TRAFFIC_SOURCE,
+            "kernel_ms": head["fused_kernel_ms"],
             "flop_per_event": flops_per_event(D, True),
             "f64_kernel": {
                 "achieved": flop / (results["f64"]["fused_kernel_ms"] * 1e-3) / 1e12,
                 "peak": PEAK["f64"],
+                "traffic": TRAFFIC["f64"],
             },
         },
     }

@@ -242,6 +242,20 @@ def latency_report(finished):
     lat = np.array([r.latency for r in finished if r.latency is not None])
     if lat.size == 0:
         return {}
+    classes = collections.defaultdict(list)
+    for r in finished:
+        classes[f"{type(r.integrand).__name__}/d{r.n_dim}/{r.target:g}"].append(r)
+    by_class = {}
+    for key, rs in sorted(classes.items()):
+        cl = np.array([r.latency for r in rs if r.latency is not None])
+        by_class[key] = {
+            "requests": len(rs),
+            "events_per_iteration": int(np.median([r.n_events for r in rs])),
+            "p50_ms": float(np.percentile(cl, 50) * 1e3) if cl.size else None,
+            "p99_ms": float(np.percentile(cl, 99) * 1e3) if cl.size else None,
+            "mean_iterations": float(np.mean([r.iterations for r in rs])),
+            "met_target": float(np.mean([r.error <= r.target * abs(r.result) for r in rs])),
+        }
     return {
         "requests": int(lat.size),
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
@@ -249,4 +263,5 @@ def latency_report(finished):
         "max_ms": float(lat.max() * 1e3),
         "mean_iterations": float(np.mean([r.iterations for r in finished])),
         "met_target": float(np.mean([r.error <= r.target * abs(r.result) for r in finished])),
+        "by_class": by_class,
     }
