@@ -23,7 +23,7 @@ namespace mv {
 template <bool MIXED, int FAM, bool TRAIN>
 __device__ __forceinline__ void batch_events(const mv_batch_item& it, const mv_batch_tile& tile,
                                              uint32_t draw, const void* tab, const void* tw,
-                                             double* hist, int nb, double* row) {
+                                             const float* cj, double* hist, int nb, double* row) {
   using T = typename std::conditional<MIXED, float, double>::type;
   Integrand<T, FAM> F(it.fn);
   const Key key{(uint32_t)(it.seed & 0xffffffffu), (uint32_t)(it.seed >> 32)};
@@ -40,8 +40,7 @@ __device__ __forceinline__ void batch_events(const mv_batch_item& it, const mv_b
     double tmp;
     if constexpr (MIXED)
       tmp = event_mixed<FAM, TRAIN>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
-                                    static_cast<const float2*>(tab), static_cast<const float*>(tw), wh,
-                                    nullptr, wscale);
+                                    static_cast<const float2*>(tab), cj, wh, nullptr, wscale);
     else
       tmp = event_f64<FAM, TRAIN>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
                                   static_cast<const double2*>(tab), static_cast<const double*>(tw), wh,
@@ -61,18 +60,23 @@ __device__ __forceinline__ void batch_tile(const mv_batch_item& it, const mv_bat
   const int d = it.n_dim;
   const int nb = d * kBins;
   T2* tab = reinterpret_cast<T2*>(smem);
-  T* tw = reinterpret_cast<T*>(smem + (MIXED ? nbmax : 2 * nbmax));
-  double* hist = smem + (MIXED ? nbmax + (nbmax + 1) / 2 : 3 * nbmax);
+  // the mixed table has no W column (fused_event.hpp)
+  T* tw = MIXED ? nullptr : reinterpret_cast<T*>(smem + 2 * nbmax);
+  double* hist = smem + (MIXED ? nbmax : 3 * nbmax);
   __syncthreads();  // the previous tile is done with the LDS
   build_table<T, FAM>(it.fn, grids + it.grid_offset, d, nullptr, nullptr, tab, tw);
   if (it.train)
     for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+  float cj[MV_FUSED_MAX_DIM];
+#pragma unroll
+  for (int j = 0; j < MV_FUSED_MAX_DIM; ++j)
+    cj[j] = MIXED && j < d ? jacobian_scale<FAM>(it.fn, j, nullptr) : 0.0f;
   __syncthreads();
   double* row = rows + it.row_offset;
   if (it.train)
-    batch_events<MIXED, FAM, true>(it, tile, draw, tab, tw, hist, nb, row);
+    batch_events<MIXED, FAM, true>(it, tile, draw, tab, tw, cj, hist, nb, row);
   else
-    batch_events<MIXED, FAM, false>(it, tile, draw, tab, tw, hist, nb, row);
+    batch_events<MIXED, FAM, false>(it, tile, draw, tab, tw, cj, hist, nb, row);
 }
 
 // FAM >= 0: every tile belongs to an item of that family (smaller register
@@ -168,7 +172,7 @@ extern "C" int mv_batch_fused(const mv_batch_item* items, int32_t n_items, const
   const int nbmax = max_dim * kBins;
   hipStream_t s = as_stream(stream);
   const bool mixed = precision == MV_MIXED;
-  const size_t lds = mixed ? (size_t)(nbmax + (nbmax + 1) / 2 + kWaves * nbmax) * sizeof(double)
+  const size_t lds = mixed ? (size_t)(nbmax + kWaves * nbmax) * sizeof(double)
                            : (size_t)(3 * nbmax + kWaves * nbmax) * sizeof(double);
 #define MV_BF(MX, F)                                                                                \
   hipLaunchKernelGGL((batch_fused<MX, F>), dim3(blocks), dim3(kBlock), lds, s, items, tiles, n_tiles, \

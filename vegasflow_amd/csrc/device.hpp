@@ -116,6 +116,7 @@ struct Integrand<T, MV_FN_CONSTANT> {
   T c;
   __device__ explicit Integrand(const mv_integrand& p) : c((T)p.comp[0][0]) {}
   __device__ static void fold(const mv_integrand&, int, double&, double&) {}
+  __device__ static double fold_scale(const mv_integrand&, int) { return 1.0; }
   __device__ __forceinline__ void begin() {}
   __device__ __forceinline__ void add(int, T) {}
   __device__ __forceinline__ void add_folded(int, T) {}
@@ -127,6 +128,7 @@ struct Integrand<T, MV_FN_PRODUCT> {
   T p;
   __device__ explicit Integrand(const mv_integrand&) : p(1) {}
   __device__ static void fold(const mv_integrand&, int, double&, double&) {}
+  __device__ static double fold_scale(const mv_integrand&, int) { return 1.0; }
   __device__ __forceinline__ void begin() { p = (T)1; }
   __device__ __forceinline__ void add(int, T x) { p *= x; }
   __device__ __forceinline__ void add_folded(int j, T x) { add(j, x); }
@@ -143,6 +145,8 @@ struct Integrand<T, MV_FN_GAUSSIAN> {
     P = (P - p.centre[0][j]) * p.comp[0][1];
     Q = Q * p.comp[0][1];
   }
+  // the factor fold() applies to Q
+  __device__ static double fold_scale(const mv_integrand& p, int) { return p.comp[0][1]; }
   __device__ __forceinline__ void begin() { coef = (T)0; }
   __device__ __forceinline__ void add(int j, T x) {
     const T t = (x - (T)prm->centre[0][j]) * inv_a;
@@ -167,6 +171,7 @@ struct Integrand<T, MV_FN_GAUSSMIX> {
   T coef[MV_MAX_COMP];
   __device__ explicit Integrand(const mv_integrand& p) : prm(&p), nc(p.n_comp) {}
   __device__ static void fold(const mv_integrand&, int, double&, double&) {}
+  __device__ static double fold_scale(const mv_integrand&, int) { return 1.0; }
   __device__ __forceinline__ void add_folded(int j, T x) { add(j, x); }
   __device__ __forceinline__ void begin() {
 #pragma unroll
@@ -208,6 +213,7 @@ struct Integrand<T, MV_FN_BWPOWER> {
   T bw, pw;
   __device__ explicit Integrand(const mv_integrand& p) : prm(&p), bw(1), pw(1) {}
   __device__ static void fold(const mv_integrand&, int, double&, double&) {}
+  __device__ static double fold_scale(const mv_integrand&, int) { return 1.0; }
   __device__ __forceinline__ void add_folded(int j, T x) { add(j, x); }
   __device__ __forceinline__ void begin() {
     bw = (T)1;

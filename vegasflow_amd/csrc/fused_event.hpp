@@ -3,8 +3,11 @@
 //
 // Per (dimension, bin) the LDS table holds the map frac -> y = P + Q frac of
 // the integrand coordinate (Integrand::fold, integration limits folded in)
-// and the Jacobian factor W = dx * 50 (vflow.py:73, 78). The FP32 table of
-// the mixed path takes 1 + frac (mixed_position) and so stores (P - Q, Q).
+// and, in FP64, the Jacobian factor W = dx * 50 (vflow.py:73, 78). The FP32
+// table of the mixed path takes 1 + frac (mixed_position), so it stores
+// (P - Q, Q), and has no W column: W = Q * c_j with the per-dimension
+// constant c_j = 50 / (xdelta_j * fold scale_j) (jacobian_scale), one
+// multiply in registers instead of one more LDS read per dimension.
 #pragma once
 
 #include <type_traits>
@@ -33,8 +36,15 @@ __device__ __forceinline__ void build_table(const mv_integrand& fn, const double
     Integrand<T, FAM>::fold(fn, j, P, Q);
     tab[i].x = sizeof(T) == 4 ? (T)(P - Q) : (T)P;
     tab[i].y = (T)Q;
-    tw[i] = (T)(dx * kFBins);  // vflow.py:78
+    if (tw) tw[i] = (T)(dx * kFBins);  // vflow.py:78
   }
+}
+
+// c_j of the mixed path's Jacobian: W = dx * 50 = Q * c_j.
+template <int FAM>
+__device__ __forceinline__ float jacobian_scale(const mv_integrand& fn, int j, const double* xdelta) {
+  const double xd = xdelta ? xdelta[j] : 1.0;
+  return (float)(kFBins / (xd * Integrand<float, FAM>::fold_scale(fn, j)));
 }
 
 // One event, FP64 (vflow.py:87-126 + 33-83 + 389-430). Returns tmp = w f
@@ -93,7 +103,7 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
 template <int FAM, bool TRAIN>
 __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
                                               uint32_t draw, Key key, int groups, int d,
-                                              const float2* tab, const float* tw, double* wh,
+                                              const float2* tab, const float* cj, double* wh,
                                               const uint32_t* inj, double wscale) {
   F.begin();
   float w = 1.0f;
@@ -111,7 +121,7 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
           const int ind = mixed_position(inj ? inj[j] : word(h, q), f1);
           const float2 t = tab[j * kBins + ind];
           F.add_folded(j, fmaf(t.y, f1, t.x));
-          w = w * tw[j * kBins + ind];
+          w = w * (t.y * cj[j]);
           if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
         }
       }

@@ -92,12 +92,14 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_mixed(StepK s, mv_integran
   extern __shared__ double smem[];
   const int d = s.n_dim;
   const int nb = d * kBins;
-  float2* tab = reinterpret_cast<float2*>(smem);    // nb float2
-  float* tw = reinterpret_cast<float*>(smem + nb);  // nb float (nb/2 doubles)
-  double* hist = smem + nb + (nb + 1) / 2;
-  build_table<float, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, tw);
+  float2* tab = reinterpret_cast<float2*>(smem);  // nb float2
+  double* hist = smem + nb;
+  build_table<float, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, nullptr);
   if (TRAIN)
     for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+  float cj[MV_FUSED_MAX_DIM];
+#pragma unroll
+  for (int j = 0; j < MV_FUSED_MAX_DIM; ++j) cj[j] = j < d ? jacobian_scale<FAM>(fn, j, s.xdelta) : 0.0f;
   __syncthreads();
   double* wh = hist + (threadIdx.x >> 6) * nb;
   Integrand<float, FAM> F(fn);
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_mixed(StepK s, mv_integran
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
     const double tmp = event_mixed<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id, s.key,
-                                               s.groups, d, tab, tw, wh, inj ? inj + i * d : nullptr,
+                                               s.groups, d, tab, cj, wh, inj ? inj + i * d : nullptr,
                                                wscale);
     acc += tmp;
     acc2 += tmp * tmp;
@@ -468,7 +470,7 @@ int launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool 
     return train ? launch_producer(vegas_fused_f64<FAM, true>, n, lds, st, k, fn, grid, R)
                  : launch_producer(vegas_fused_f64<FAM, false>, n, lds, st, k, fn, grid, R);
   }
-  const size_t lds = (size_t)(nb + (nb + 1) / 2 + (train ? kWaves * nb : 0)) * sizeof(double);
+  const size_t lds = (size_t)(nb + (train ? kWaves * nb : 0)) * sizeof(double);
   return train ? launch_producer(vegas_fused_mixed<FAM, true>, n, lds, st, k, fn, grid, R)
                : launch_producer(vegas_fused_mixed<FAM, false>, n, lds, st, k, fn, grid, R);
 }
