@@ -136,8 +136,11 @@ def bench_unfused(vf, dtype, steps=10):
     torch.cuda.synchronize()
     ms = [sum(m[i].elapsed_time(m[i + 1]) for m in marks) / steps for i in range(3)]
     b = unfused_bytes(D, s)
+    inst.run_integration(1)  # allocations of the integrand's temporaries
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     res, sig = inst.run_integration(5)
+    torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return {"events_per_s_run_integration": 5 * N / dt, "integral_5_iterations": float(res),
             "sigma": float(sig),

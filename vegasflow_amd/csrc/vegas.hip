@@ -167,114 +167,168 @@ __global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, 
 
 // ---------------------------------------------------------------------------
 // unfused path: sample (any n_dim)
+// With STAGED, each block writes its 256 events' rows of x (and bins) to LDS
+// first and then stores the block's contiguous part of the row-major (n, d)
+// outputs with consecutive threads on consecutive elements (full cache lines
+// instead of one element per thread every d elements). STAGED needs
+// stage_bytes(d) of LDS; beyond that the kernels store directly.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void vegas_sample_f64(StepK s, const double* __restrict__ grid,
-                                                           double* __restrict__ xo,
-                                                           double* __restrict__ wo,
-                                                           uint8_t* __restrict__ io) {
-  const int d = s.n_dim;
-  const bool limits = s.xmin != nullptr;
-  const double* inj = static_cast<const double*>(s.inject);
-  const int64_t step = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
-    double w = 1.0;
-    for (int g = 0; g < s.groups; ++g) {
-      uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
-      if (!inj) {
-        h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-        l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int j = 4 * g + q;
-        if (j < d) {
-          const double u = inj ? inj[i * d + j] : draw_f64(word(h, q), word(l, q));
-          const double xn = kFBins * (1.0 - u);
-          const int ind = clamp_bin((int)xn);
-          const double lo = grid[j * (kBins + 1) + ind];
-          const double dx = grid[j * (kBins + 1) + ind + 1] - lo;
-          const double frac = xn - floor(xn);
-          double x = lo + dx * frac;  // two roundings, as vflow.py:76
-          w = w * (dx * kFBins);
-          if (limits) x = s.xmin[j] + x * s.xdelta[j];
-          xo[i * d + j] = x;
-          if (io) io[i * d + j] = (uint8_t)ind;
-        }
-      }
-    }
-    double wt = w * s.xjac;
-    if (limits) wt = wt * s.xdeltajac;
-    wo[i] = wt;
-  }
+constexpr size_t kStageMax = 64 * 1024;
+
+inline size_t stage_bytes(int d, size_t elem, bool bins) {
+  return (size_t)kBlock * d * elem + (bins ? (((size_t)kBlock * d + 3) & ~(size_t)3) : 0);
 }
 
-__global__ __launch_bounds__(kBlock) void vegas_sample_mixed(StepK s, const double* __restrict__ grid,
-                                                             float* __restrict__ xo,
-                                                             float* __restrict__ wo,
-                                                             uint8_t* __restrict__ io) {
-  const int d = s.n_dim;
-  const bool limits = s.xmin != nullptr;
-  const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
-  const float wscale = (float)(limits ? s.xjac * s.xdeltajac : s.xjac);
-  const int64_t step = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
-    float w = 1.0f;
-    for (int g = 0; g < s.groups; ++g) {
-      uint4 h = make_uint4(0, 0, 0, 0);
-      if (!inj) h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int j = 4 * g + q;
-        if (j < d) {
-          float f1;
-          const int ind = mixed_position(inj ? inj[i * d + j] : word(h, q), f1);
-          const float frac = f1 - 1.0f;  // exact
-          const double lo = grid[j * (kBins + 1) + ind];
-          const double dx = grid[j * (kBins + 1) + ind + 1] - lo;
-          float x = fmaf((float)dx, frac, (float)lo);
-          w = w * (float)(dx * kFBins);
-          if (limits) x = fmaf(x, (float)s.xdelta[j], (float)s.xmin[j]);
-          xo[i * d + j] = x;
-          if (io) io[i * d + j] = (uint8_t)ind;
-        }
-      }
-    }
-    wo[i] = w * wscale;
-  }
+template <typename T>
+__device__ __forceinline__ void stage_store(const T* st, T* __restrict__ dst, int elems) {
+  for (int k = threadIdx.x; k < elems; k += kBlock) dst[k] = st[k];
 }
 
-template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void plain_sample(StepK s, void* __restrict__ xo_) {
+__device__ __forceinline__ void stage_store_bytes(const uint8_t* st, uint8_t* __restrict__ dst,
+                                                  int elems) {
+  int done = 0;
+  if (((uintptr_t)dst & 3u) == 0) {  // st is 4-aligned (LDS, after the x rows)
+    const int words = elems >> 2;
+    const uint32_t* s4 = reinterpret_cast<const uint32_t*>(st);
+    uint32_t* d4 = reinterpret_cast<uint32_t*>(dst);
+    for (int k = threadIdx.x; k < words; k += kBlock) d4[k] = s4[k];
+    done = words << 2;
+  }
+  for (int k = done + threadIdx.x; k < elems; k += kBlock) dst[k] = st[k];
+}
+
+// vflow.py:87-126 + 33-83, monte_carlo.py:249-275. FP64: the reference's
+// operation order (x = lo + dx * frac with two roundings). Mixed: the
+// positions of mixed_position, map in FP32.
+template <bool MIXED, bool STAGED>
+__global__ __launch_bounds__(kBlock) void vegas_sample(StepK s, const double* __restrict__ grid,
+                                                       void* __restrict__ xo_, void* __restrict__ wo_,
+                                                       uint8_t* __restrict__ io) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  extern __shared__ double stage_mem[];
+  T* xo = static_cast<T*>(xo_);
+  T* wo = static_cast<T*>(wo_);
   const int d = s.n_dim;
+  T* xs = reinterpret_cast<T*>(stage_mem);
+  uint8_t* bs = reinterpret_cast<uint8_t*>(xs + (size_t)kBlock * d);
   const bool limits = s.xmin != nullptr;
+  const int64_t n = s.n_events;
   const int64_t step = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
-    for (int g = 0; g < s.groups; ++g) {
-      uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
-      if (!s.inject) {
-        h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-        if (!MIXED) l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
-      }
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += step) {  // block-uniform
+    const int64_t i = base + threadIdx.x;
+    if (i < n) {
+      const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
+      T w = (T)1;
+      for (int g = 0; g < s.groups; ++g) {
+        uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
+        if (!s.inject) {
+          h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+          if (!MIXED) l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+        }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int j = 4 * g + q;
-        if (j < d) {
-          if (MIXED) {
-            const uint32_t hw = s.inject ? static_cast<const uint32_t*>(s.inject)[i * d + j]
-                                         : word(h, q);
-            float x = fmaf((float)(hw >> 9), 0x1p-23f, 0x1p-24f);
-            if (limits) x = fmaf(x, (float)s.xdelta[j], (float)s.xmin[j]);
-            static_cast<float*>(xo_)[i * d + j] = x;
-          } else {
-            const double u = s.inject ? static_cast<const double*>(s.inject)[i * d + j]
-                                      : draw_f64(word(h, q), word(l, q));
-            static_cast<double*>(xo_)[i * d + j] = limits ? s.xmin[j] + u * s.xdelta[j] : u;
+        for (int q = 0; q < 4; ++q) {
+          const int j = 4 * g + q;
+          if (j < d) {
+            int ind;
+            T x;
+            if constexpr (MIXED) {
+              const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
+              float f1;
+              ind = mixed_position(inj ? inj[i * d + j] : word(h, q), f1);
+              const double lo = grid[j * (kBins + 1) + ind];
+              const double dx = grid[j * (kBins + 1) + ind + 1] - lo;
+              x = fmaf((float)dx, f1 - 1.0f, (float)lo);  // f1 - 1 is exact
+              w = w * (float)(dx * kFBins);
+              if (limits) x = fmaf(x, (float)s.xdelta[j], (float)s.xmin[j]);
+            } else {
+              const double* inj = static_cast<const double*>(s.inject);
+              const double u = inj ? inj[i * d + j] : draw_f64(word(h, q), word(l, q));
+              const double xn = kFBins * (1.0 - u);  // vflow.py:117
+              ind = clamp_bin((int)xn);               // vflow.py:67
+              const double lo = grid[j * (kBins + 1) + ind];
+              const double dx = grid[j * (kBins + 1) + ind + 1] - lo;  // vflow.py:73
+              const double frac = xn - floor(xn);     // vflow.py:75
+              x = lo + dx * frac;                     // two roundings, as vflow.py:76
+              w = w * (dx * kFBins);                  // vflow.py:78
+              if (limits) x = s.xmin[j] + x * s.xdelta[j];  // monte_carlo.py:273
+            }
+            if (STAGED) {
+              xs[threadIdx.x * d + j] = x;
+              if (io) bs[threadIdx.x * d + j] = (uint8_t)ind;
+            } else {
+              xo[i * d + j] = x;
+              if (io) io[i * d + j] = (uint8_t)ind;
+            }
           }
         }
       }
+      if constexpr (MIXED) {
+        wo[i] = w * (float)(limits ? s.xjac * s.xdeltajac : s.xjac);
+      } else {
+        double wt = w * s.xjac;  // monte_carlo.py:270
+        if (limits) wt = wt * s.xdeltajac;
+        wo[i] = wt;
+      }
+    }
+    if (STAGED) {
+      __syncthreads();
+      const int elems = (int)(n - base < kBlock ? n - base : kBlock) * d;
+      stage_store(xs, xo + base * d, elems);
+      if (io) stage_store_bytes(bs, io + base * d, elems);
+      __syncthreads();
+    }
+  }
+}
+
+// monte_carlo.py:264-274 (PlainFlow draws): x = u, or xmin + u * xdelta.
+template <bool MIXED, bool STAGED>
+__global__ __launch_bounds__(kBlock) void plain_sample(StepK s, void* __restrict__ xo_) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  extern __shared__ double stage_mem[];
+  T* xo = static_cast<T*>(xo_);
+  T* xs = reinterpret_cast<T*>(stage_mem);
+  const int d = s.n_dim;
+  const bool limits = s.xmin != nullptr;
+  const int64_t n = s.n_events;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += step) {  // block-uniform
+    const int64_t i = base + threadIdx.x;
+    if (i < n) {
+      const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
+      for (int g = 0; g < s.groups; ++g) {
+        uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
+        if (!s.inject) {
+          h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
+          if (!MIXED) l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = 4 * g + q;
+          if (j < d) {
+            T x;
+            if constexpr (MIXED) {
+              const uint32_t hw = s.inject ? static_cast<const uint32_t*>(s.inject)[i * d + j]
+                                           : word(h, q);
+              float xf = fmaf((float)(hw >> 9), 0x1p-23f, 0x1p-24f);
+              if (limits) xf = fmaf(xf, (float)s.xdelta[j], (float)s.xmin[j]);
+              x = xf;
+            } else {
+              const double u = s.inject ? static_cast<const double*>(s.inject)[i * d + j]
+                                        : draw_f64(word(h, q), word(l, q));
+              x = limits ? s.xmin[j] + u * s.xdelta[j] : u;
+            }
+            if (STAGED)
+              xs[threadIdx.x * d + j] = x;
+            else
+              xo[i * d + j] = x;
+          }
+        }
+      }
+    }
+    if (STAGED) {
+      __syncthreads();
+      stage_store(xs, xo + base * d, (int)(n - base < kBlock ? n - base : kBlock) * d);
+      __syncthreads();
     }
   }
 }
@@ -553,15 +607,21 @@ extern "C" int mv_vegas_sample(const mv_step* step, const double* grid, void* x,
   hipStream_t st = as_stream(stream);
   const bool f64 = step->precision == MV_F64;
   const int64_t d = step->n_dim;
-  for_each_counter_range(*step, f64 ? 8 : 4, [&](const StepK& k, int64_t e0) {
+  const size_t elem = f64 ? 8 : 4;
+  const size_t lds = stage_bytes(step->n_dim, elem, idx != nullptr);
+  const bool staged = lds <= kStageMax;
+  for_each_counter_range(*step, elem, [&](const StepK& k, int64_t e0) {
     const int blocks = sample_blocks(k.n_events);
     uint8_t* io = idx ? idx + e0 * d : nullptr;
-    if (f64)
-      hipLaunchKernelGGL(vegas_sample_f64, dim3(blocks), dim3(kBlock), 0, st, k, grid,
-                         static_cast<double*>(x) + e0 * d, static_cast<double*>(w) + e0, io);
-    else
-      hipLaunchKernelGGL(vegas_sample_mixed, dim3(blocks), dim3(kBlock), 0, st, k, grid,
-                         static_cast<float*>(x) + e0 * d, static_cast<float*>(w) + e0, io);
+    void* xp = static_cast<char*>(x) + e0 * d * (int64_t)elem;
+    void* wp = static_cast<char*>(w) + e0 * (int64_t)elem;
+    if (f64) {
+      if (staged) hipLaunchKernelGGL((vegas_sample<false, true>), dim3(blocks), dim3(kBlock), lds, st, k, grid, xp, wp, io);
+      else hipLaunchKernelGGL((vegas_sample<false, false>), dim3(blocks), dim3(kBlock), 0, st, k, grid, xp, wp, io);
+    } else {
+      if (staged) hipLaunchKernelGGL((vegas_sample<true, true>), dim3(blocks), dim3(kBlock), lds, st, k, grid, xp, wp, io);
+      else hipLaunchKernelGGL((vegas_sample<true, false>), dim3(blocks), dim3(kBlock), 0, st, k, grid, xp, wp, io);
+    }
     return blocks;
   });
   return check_launch("mv_vegas_sample");
@@ -575,14 +635,19 @@ extern "C" int mv_plain_sample(const mv_step* step, void* x, void* stream) {
   hipStream_t st = as_stream(stream);
   const bool f64 = step->precision == MV_F64;
   const int64_t d = step->n_dim;
-  for_each_counter_range(*step, f64 ? 8 : 4, [&](const StepK& k, int64_t e0) {
+  const size_t elem = f64 ? 8 : 4;
+  const size_t lds = stage_bytes(step->n_dim, elem, false);
+  const bool staged = lds <= kStageMax;
+  for_each_counter_range(*step, elem, [&](const StepK& k, int64_t e0) {
     const int blocks = sample_blocks(k.n_events);
-    if (f64)
-      hipLaunchKernelGGL(plain_sample<false>, dim3(blocks), dim3(kBlock), 0, st, k,
-                         static_cast<void*>(static_cast<double*>(x) + e0 * d));
-    else
-      hipLaunchKernelGGL(plain_sample<true>, dim3(blocks), dim3(kBlock), 0, st, k,
-                         static_cast<void*>(static_cast<float*>(x) + e0 * d));
+    void* xp = static_cast<char*>(x) + e0 * d * (int64_t)elem;
+    if (f64) {
+      if (staged) hipLaunchKernelGGL((plain_sample<false, true>), dim3(blocks), dim3(kBlock), lds, st, k, xp);
+      else hipLaunchKernelGGL((plain_sample<false, false>), dim3(blocks), dim3(kBlock), 0, st, k, xp);
+    } else {
+      if (staged) hipLaunchKernelGGL((plain_sample<true, true>), dim3(blocks), dim3(kBlock), lds, st, k, xp);
+      else hipLaunchKernelGGL((plain_sample<true, false>), dim3(blocks), dim3(kBlock), 0, st, k, xp);
+    }
     return blocks;
   });
   return check_launch("mv_plain_sample");
