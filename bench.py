@@ -319,6 +319,8 @@ def main():
     ap.add_argument("--no-protocol", action="store_true", help="skip the 5+5 iteration error check")
     ap.add_argument("--no-extra", action="store_true", help="skip the VEGAS+, batched and serving runs")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the Poisson stream")
+    ap.add_argument("--only", choices=["unfused", "vplus", "batch", "serve"],
+                    help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1",
@@ -332,6 +334,18 @@ def main():
     torch.cuda.set_device(0)
     if args.calibrate_copy:
         print(json.dumps(calibrate_copy(torch)))
+        return 0
+    precisions = (("mixed", torch.float32), ("f64", torch.float64))
+    if args.only:
+        if args.only == "unfused":
+            out = {p: bench_unfused(vf, dt_) for p, dt_ in precisions}
+        elif args.only == "vplus":
+            out = {p: bench_vplus(vf, dt_) for p, dt_ in precisions}
+        elif args.only == "batch":
+            out = {p: bench_batched(torch, dt_) for p, dt_ in precisions}
+        else:
+            out = bench_serving(args.serve_seconds)
+        print(json.dumps({"section": args.only, "result": out}))
         return 0
     fn = vf.Gaussian(A, 0.5)
     exact = fn.exact(D)

@@ -216,8 +216,12 @@ int mv_vplus_sample(const mv_step* step, const mv_strata* strata, const double* 
 int mv_vplus_digest(const mv_step* step, const mv_strata* strata, const void* f,
                     const void* w, const uint8_t* idx, const int32_t* seg, double* ress,
                     double* ress2, const mv_partials* parts, void* stream);
-/* Iteration end (vflowplus.py:222-242, 153-163): out = {res, sigma,
- * events of the next iteration}; when adaptive, n_ev and offsets are
+/* Doubles of ``out`` for mv_vplus_finalize: 3 results, then 4 doubles per
+ * 256 cubes that the call uses as scratch. */
+int64_t mv_vplus_out_len(int32_t n_cubes);
+/* Iteration end (vflowplus.py:222-242, 153-163): out (mv_vplus_out_len
+ * doubles) = {res, sigma, events of the next iteration, scratch}; when
+ * adaptive, n_ev and offsets are
  * rewritten for the next iteration (damped = max(arr_var, 0)^(beta/2),
  * n_ev = int(max(min_neval, damped * init_calls / 2 / sum damped))); ress
  * and ress2 are zeroed; with train the histogram rows are reduced into

@@ -178,3 +178,41 @@ def test_vplus_mixed_tracks_f64():
         res[prec] = out.cpu().numpy()
     assert abs(res["mixed"][0] - res["f64"][0]) < 0.05 * res["f64"][1]
     assert abs(res["mixed"][1] - res["f64"][1]) < 0.01 * res["f64"][1]
+
+
+@pytest.mark.parametrize("n_cubes", [7, 10000, 14641])
+def test_vplus_finalize_matches_closed_form(n_cubes):
+    """mv_vplus_finalize alone on synthetic per-cube sums (beyond 10240
+    cubes a thread takes several chunks): sigma, the reallocation
+    (vflowplus.py:153-163), offsets and zeroed accumulators."""
+    rng = np.random.default_rng(n_cubes)
+    n_strat = {7: 7, 10000: 10, 14641: 11}[n_cubes]
+    d = {7: 1, 10000: 4, 14641: 4}[n_cubes]
+    ne = rng.integers(2, 90, size=n_cubes).astype(np.int32)
+    r = rng.random(n_cubes) * 1e-4
+    r2 = (r ** 2 / ne) * (1.0 + rng.random(n_cubes))
+    r2[::13] = r[::13] ** 2 / ne[::13] * 0.999  # some negative variances (clamped)
+    n_ev = dev(ne, torch.int32)
+    off = np.zeros(n_cubes + 1, dtype=np.int64)
+    off[1:] = np.cumsum(ne)
+    offsets = dev(off, torch.int32)
+    strata = kernels.make_strata(n_strat, n_cubes, n_ev, offsets)
+    ress, ress2 = dev(r), dev(r2)
+    ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, True))
+    arr_var = torch.empty(n_cubes, dtype=torch.float64, device=DEV)
+    init_calls, min_neval = 1_000_000, 2
+    out = kernels.vplus_finalize(d, strata, n_ev, offsets, ress, ress2, True, min_neval, init_calls, ora.BETA,
+                                 ws, 1, False, False, None, arr_var).cpu().numpy()
+    var = r2 * ne - r ** 2
+    np.testing.assert_allclose(arr_var.cpu().numpy(), var, rtol=1e-12, atol=1e-30)
+    assert rel(out[0], r.sum()) < 1e-12
+    assert rel(out[1], np.sqrt(np.sum(np.maximum(var, 0.0) / (ne - 1.0)))) < 1e-12
+    want = ora.redistribute_samples(var, min_neval, init_calls)
+    got = n_ev.cpu().numpy()
+    diff = np.abs(got.astype(np.int64) - want.astype(np.int64))
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    o = offsets.cpu().numpy()
+    assert o[0] == 0
+    np.testing.assert_array_equal(o[1:], np.cumsum(got))
+    assert int(out[2]) == int(got.sum())
+    assert float(ress.abs().max()) == 0.0 and float(ress2.abs().max()) == 0.0

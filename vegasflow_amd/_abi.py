@@ -144,6 +144,7 @@ SIGNATURES = {
     "mv_vplus_sample": (ctypes.c_int, [_P(Step), _P(Strata), _vp, _vp, _vp, _vp, _vp, _vp]),
     "mv_vplus_digest": (ctypes.c_int, [_P(Step), _P(Strata), _vp, _vp, _vp, _vp, _vp, _vp,
                                        _P(Partials), _vp]),
+    "mv_vplus_out_len": (_i64, [_i32]),
     "mv_vplus_finalize": (ctypes.c_int, [_i32, _P(Strata), _vp, _vp, _vp, _vp, _i32, _i32, _f64,
                                          _f64, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
                                          _vp]),

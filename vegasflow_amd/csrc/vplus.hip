@@ -334,90 +334,166 @@ __global__ __launch_bounds__(kBlock) void vplus_digest(StepK s, StrataK st, cons
 // iteration end over the cubes (vflowplus.py:222-242 + 153-163), one
 // workgroup of 1024 threads; thread t owns a contiguous chunk of cubes.
 // ---------------------------------------------------------------------------
-constexpr int kCubeThreads = 1024;
+// VEGAS+ iteration end over the cubes, in three launches of kBlock-thread
+// workgroups with one thread per cube (coalesced, spread over the device):
+// vplus_cube_stats, vplus_realloc, vplus_offsets. ``out`` carries the
+// results and, after them, the per-workgroup scratch (3 + 4 nb doubles,
+// nb = ceil(n_cubes / kBlock)): out[3 + 3 b + k] partial sums, out[3 + 3 nb
+// + b] event counts per workgroup.
 
-__device__ __forceinline__ double block_sum_1024(double v, double* red) {
-  red[threadIdx.x] = v;
+// Sum of v over the kBlock threads, wave totals added in order.
+__device__ __forceinline__ double block_sum(double v, double* part) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int h = kCubeThreads / 2; h >= 1; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + h];
-    __syncthreads();
-  }
-  const double r = red[0];
+  double a = part[0];
+  for (int k = 1; k < kWaves; ++k) a += part[k];
   __syncthreads();
-  return r;
+  return a;
 }
 
-__global__ __launch_bounds__(kCubeThreads) void vplus_cubes_kernel(
-    int n_cubes, int32_t* __restrict__ n_ev, int32_t* __restrict__ off, double* __restrict__ ress,
-    double* __restrict__ ress2, int adaptive, int min_neval, double init_calls, double beta,
-    double* __restrict__ out, double* __restrict__ arr_var_out) {
-  __shared__ double red[kCubeThreads];
-  __shared__ int iscan[kCubeThreads];
-  const int t = threadIdx.x;
-  const int per = (n_cubes + kCubeThreads - 1) / kCubeThreads;
-  const int c0 = min(n_cubes, t * per), c1 = min(n_cubes, c0 + per);
-  double rs = 0.0, s2 = 0.0;
-  for (int c = c0; c < c1; ++c) {
+// Exclusive prefix sum of v over the kBlock threads; *total = the sum.
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
+  __shared__ int64_t wtot[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) wtot[w] = inc;
+  __syncthreads();
+  int64_t before = 0, all = 0;
+  for (int k = 0; k < kWaves; ++k) {
+    before += k < w ? wtot[k] : 0;
+    all += wtot[k];
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - v;
+}
+
+// Part 1 (vflowplus.py:216-233, 157): per cube the variance
+// ress2 * n_ev - ress^2 (to arr_var_out) and, when adaptive, the damped
+// variance max(var, 0)^(beta/2), left in ress2 for part 2; per workgroup the
+// partial sums of ress, max(var, 0) / (n_ev - 1) and the damped variances.
+__global__ __launch_bounds__(kBlock) void vplus_cube_stats(int n_cubes, const int32_t* __restrict__ n_ev,
+                                                           const double* __restrict__ ress,
+                                                           double* __restrict__ ress2, int adaptive,
+                                                           double beta, double* __restrict__ out,
+                                                           double* __restrict__ arr_var_out) {
+  __shared__ double part[kWaves];
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  double rs = 0.0, s2 = 0.0, dl = 0.0;
+  if (c < n_cubes) {
     const double fn = (double)n_ev[c];
     const double r = ress[c];
     const double var = ress2[c] * fn - r * r;  // vflowplus.py:216-217
     if (arr_var_out) arr_var_out[c] = var;
-    rs += r;                                   // vflowplus.py:231
-    s2 += fmax(var, 0.0) / (fn - 1.0);         // vflowplus.py:230, 232
-  }
-  const double res = block_sum_1024(rs, red);
-  const double sig2 = block_sum_1024(s2, red);
-  if (adaptive) {
-    // damped = arr_var^(beta/2) (vflowplus.py:157), clamped at zero first
-    double dl = 0.0;
-    for (int c = c0; c < c1; ++c) {
-      const double fn = (double)n_ev[c];
-      const double r = ress[c];
-      dl += pow(fmax(ress2[c] * fn - r * r, 0.0), 0.5 * beta);
-    }
-    const double dsum = block_sum_1024(dl, red);
-    if (dsum > 0.0) {
-      for (int c = c0; c < c1; ++c) {
-        const double fn = (double)n_ev[c];
-        const double r = ress[c];
-        const double damped = pow(fmax(ress2[c] * fn - r * r, 0.0), 0.5 * beta);
-        const double nv = fmax((double)min_neval, damped * init_calls / 2.0 / dsum);  // 158-161
-        n_ev[c] = (int32_t)nv;                                                           // 162
-      }
+    rs = r;                                    // vflowplus.py:231
+    s2 = fmax(var, 0.0) / (fn - 1.0);          // vflowplus.py:230, 232
+    if (adaptive) {
+      dl = pow(fmax(var, 0.0), 0.5 * beta);    // vflowplus.py:157, clamped at zero first
+      ress2[c] = dl;
     }
   }
-  // offsets = exclusive prefix sum of n_ev (vflowplus.py:67)
-  int local = 0;
-  for (int c = c0; c < c1; ++c) local += n_ev[c];
-  iscan[t] = local;
-  __syncthreads();
-  for (int o = 1; o < kCubeThreads; o <<= 1) {
-    const int v = t >= o ? iscan[t - o] : 0;
-    __syncthreads();
-    iscan[t] += v;
-    __syncthreads();
-  }
-  int run = iscan[t] - local;
-  for (int c = c0; c < c1; ++c) {
-    off[c] = run;
-    run += n_ev[c];
-    ress[c] = 0.0;
-    ress2[c] = 0.0;
-  }
-  if (t == kCubeThreads - 1) {
-    off[n_cubes] = iscan[t];
-    out[0] = res;
-    out[1] = sqrt(sig2);  // vflowplus.py:233
-    out[2] = (double)iscan[t];
+  rs = block_sum(rs, part);
+  s2 = block_sum(s2, part);
+  dl = block_sum(dl, part);
+  if (threadIdx.x == 0) {
+    double* o = out + 3 + 3 * (int64_t)blockIdx.x;
+    o[0] = rs;
+    o[1] = s2;
+    o[2] = dl;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void vplus_hist_kernel(FzK z, double* __restrict__ P,
-                                                            int64_t stride, int nrows,
-                                                            double* __restrict__ grid,
-                                                            double* __restrict__ hist_out) {
-  hist_refine_block(z, P, stride, nrows, (int)blockIdx.x, grid, hist_out);
+// The partial sums of part 1 over all workgroups, in order (every workgroup
+// of parts 2 and 3 recomputes them; there are few).
+__device__ __forceinline__ void cube_totals(const double* __restrict__ out, int nb, double* tot) {
+  __shared__ double buf[3][kBlock];
+  double acc = 0.0;
+  for (int b0 = 0; b0 < nb; b0 += kBlock) {
+    if (b0 + (int)threadIdx.x < nb)
+      for (int k = 0; k < 3; ++k) buf[k][threadIdx.x] = out[3 + 3 * (int64_t)(b0 + threadIdx.x) + k];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      const int m = min(kBlock, nb - b0);
+      for (int i = 0; i < m; ++i) acc = (b0 + i == 0) ? buf[threadIdx.x][i] : acc + buf[threadIdx.x][i];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) tot[threadIdx.x] = acc;
+  __syncthreads();
+}
+
+// Part 2 (vflowplus.py:158-162): n_ev = int(max(min_neval, damped *
+// init_calls / 2 / sum damped)) when adaptive (and the sum is positive), the
+// events per workgroup, and out[0..1] = {res, sigma} (vflowplus.py:231-233).
+__global__ __launch_bounds__(kBlock) void vplus_realloc(int n_cubes, int32_t* __restrict__ n_ev,
+                                                        const double* __restrict__ ress2, int adaptive,
+                                                        int min_neval, double init_calls,
+                                                        double* __restrict__ out) {
+  __shared__ double tot[3];
+  __shared__ double part[kWaves];
+  const int nb = (n_cubes + kBlock - 1) / kBlock;
+  cube_totals(out, nb, tot);
+  const double dsum = tot[2];
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  double ne = 0.0;
+  if (c < n_cubes) {
+    int v = n_ev[c];
+    if (adaptive && dsum > 0.0) {
+      v = (int32_t)fmax((double)min_neval, ress2[c] * init_calls / 2.0 / dsum);
+      n_ev[c] = v;
+    }
+    ne = (double)v;
+  }
+  ne = block_sum(ne, part);  // an integer < 2^53: exact
+  if (threadIdx.x == 0) {
+    out[3 + 3 * (int64_t)nb + blockIdx.x] = ne;
+    if (blockIdx.x == 0) {
+      out[0] = tot[0];
+      out[1] = sqrt(tot[1]);  // vflowplus.py:233
+    }
+  }
+}
+
+// Part 3: offsets of the next iteration, the exclusive prefix sum of n_ev
+// (vflowplus.py:67), out[2] = events of the next iteration, ress and ress2
+// zeroed. Workgroups beyond the cubes' (with train): the histogram rows of one
+// dimension each and the grid refinement (vflow.py:349-362).
+__global__ __launch_bounds__(kBlock) void vplus_offsets(int n_cubes, const int32_t* __restrict__ n_ev,
+                                                        int32_t* __restrict__ off, double* __restrict__ ress,
+                                                        double* __restrict__ ress2, double* __restrict__ out,
+                                                        FzK z, double* __restrict__ P, int64_t stride,
+                                                        int nrows, double* __restrict__ grid,
+                                                        double* __restrict__ hist_out) {
+  const int nb = (n_cubes + kBlock - 1) / kBlock;
+  if ((int)blockIdx.x >= nb) {
+    hist_refine_block(z, P, stride, nrows, (int)blockIdx.x - nb, grid, hist_out);
+    return;
+  }
+  __shared__ double part[kWaves];
+  const double* bt = out + 3 + 3 * (int64_t)nb;
+  // events of the workgroups before this one (integers: order-free, exact)
+  double before = 0.0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kBlock) before += bt[b];
+  before = block_sum(before, part);
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int64_t v = c < n_cubes ? n_ev[c] : 0;
+  int64_t total = 0;
+  const int64_t ex = block_exclusive_scan(v, &total) + (int64_t)before;
+  if (c < n_cubes) {
+    off[c] = (int32_t)ex;
+    ress[c] = 0.0;
+    ress2[c] = 0.0;
+  }
+  if ((int)blockIdx.x == nb - 1 && threadIdx.x == 0) {
+    off[n_cubes] = (int32_t)((int64_t)before + total);
+    out[2] = before + (double)total;
+  }
 }
 
 }  // namespace mv
@@ -530,6 +606,10 @@ extern "C" int mv_vplus_digest(const mv_step* step, const mv_strata* strata, con
   return rows_used(blocks);
 }
 
+extern "C" int64_t mv_vplus_out_len(int32_t n_cubes) {
+  return 3 + 4 * (int64_t)((n_cubes + kBlock - 1) / kBlock);
+}
+
 extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t* n_ev, int32_t* offsets,
                                  double* ress, double* ress2, int32_t adaptive, int32_t min_neval,
                                  double init_calls, double beta, double* rows, int64_t row_stride,
@@ -540,10 +620,6 @@ extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t
   if (!n_ev || !offsets || !ress || !ress2 || !out)
     return fail(MV_ERR_ARG, "mv_vplus_finalize: missing buffer");
   if (adaptive && min_neval < 1) return fail(MV_ERR_ARG, "mv_vplus_finalize: min_neval < 1");
-  hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(vplus_cubes_kernel, dim3(1), dim3(kCubeThreads), 0, s, strata->n_cubes, n_ev,
-                     offsets, ress, ress2, adaptive, min_neval, init_calls, beta, out, arr_var_out);
-  if ((rc = check_launch("mv_vplus_finalize"))) return rc;
   if (train) {  // histogram columns are always consumed
     if (!rows || n_rows < 1 || n_rows > MV_ROWS)
       return fail(MV_ERR_ARG, "mv_vplus_finalize: histogram rows missing or n_rows %d not in 1..%d", n_rows,
@@ -551,10 +627,18 @@ extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t
     if (row_stride < mv_partial_stride(n_dim, 1, 1))
       return fail(MV_ERR_ARG, "mv_vplus_finalize: row_stride too small");
     if (refine && !grid) return fail(MV_ERR_ARG, "mv_vplus_finalize: grid required to refine");
-    FzK z{n_dim, 1, 1, refine, 0, 0.0};
-    hipLaunchKernelGGL(vplus_hist_kernel, dim3(n_dim), dim3(kBlock), 0, s, z, rows, row_stride, n_rows,
-                       grid, hist_out);
-    if ((rc = check_launch("mv_vplus_finalize"))) return rc;
   }
-  return MV_OK;
+  const int n_cubes = strata->n_cubes;
+  const int nb = (n_cubes + kBlock - 1) / kBlock;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(vplus_cube_stats, dim3(nb), dim3(kBlock), 0, s, n_cubes, n_ev, ress, ress2, adaptive,
+                     beta, out, arr_var_out);
+  if ((rc = check_launch("mv_vplus_finalize"))) return rc;
+  hipLaunchKernelGGL(vplus_realloc, dim3(nb), dim3(kBlock), 0, s, n_cubes, n_ev, ress2, adaptive, min_neval,
+                     init_calls, out);
+  if ((rc = check_launch("mv_vplus_finalize"))) return rc;
+  const FzK z{n_dim, 1, 1, refine, 0, 0.0};
+  hipLaunchKernelGGL(vplus_offsets, dim3(nb + (train ? n_dim : 0)), dim3(kBlock), 0, s, n_cubes, n_ev,
+                     offsets, ress, ress2, out, z, rows, row_stride, n_rows, grid, hist_out);
+  return check_launch("mv_vplus_finalize");
 }

@@ -265,12 +265,13 @@ def vplus_digest(step, strata, f, w, idx, seg, ress, ress2, ws):
 def vplus_finalize(n_dim, strata, n_ev, offsets, ress, ress2, adaptive, min_neval, init_calls, beta,
                    ws, n_rows, train, refine, grid, arr_var_out=None, hist_out=None):
     """Returns a (3,) float64 tensor: res, sigma, events of the next iteration."""
-    out = torch.empty(3, dtype=torch.float64, device=ws.device)
+    out = torch.empty(int(_abi.load().mv_vplus_out_len(int(strata.n_cubes))), dtype=torch.float64,
+                      device=ws.device)
     _abi.call("mv_vplus_finalize", int(n_dim), ctypes.byref(strata), _ptr(n_ev), _ptr(offsets), _ptr(ress),
               _ptr(ress2), int(bool(adaptive)), int(min_neval), float(init_calls), float(beta),
               _ptr(ws.rows), ws.stride, int(n_rows), int(bool(train)), int(bool(refine)),
               _ptr(grid), _ptr(out), _ptr(arr_var_out), _ptr(hist_out), _stream())
-    return out
+    return out[:3]
 
 
 # ---------------------------------------------------------------------------
