@@ -224,8 +224,11 @@ class VegasFlowPlus(VegasFlow):
 
     def run_integration(self, n_iter, log_time=True, histograms=None):
         result = super().run_integration(n_iter, log_time=log_time, histograms=histograms)
-        for e in self._pending_events:
-            self.events_history.append(int(e.item()) if torch.is_tensor(e) else int(e))
+        pend = self._pending_events
+        dev = [e.reshape(1) for e in pend if torch.is_tensor(e)]
+        vals = iter(torch.cat(dev).cpu().tolist() if dev else [])  # one copy for the run
+        for e in pend:
+            self.events_history.append(int(next(vals)) if torch.is_tensor(e) else int(e))
         self._pending_events = []
         if self._last_out is not None:
             self._n_events = int(self._last_out[2].item())
