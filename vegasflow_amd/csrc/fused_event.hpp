@@ -47,6 +47,16 @@ __device__ __forceinline__ float jacobian_scale(const mv_integrand& fn, int j, c
   return (float)(kFBins / (xd * Integrand<float, FAM>::fold_scale(fn, j)));
 }
 
+// Histogram adds of one event: tmp2 into bin pk[j] of every dimension j
+// (byte j & 3 of pk[j >> 2]). Spreading one wave instruction's adds over
+// several dimensions (lane-rotated order) was measured slower on gfx950: the
+// extra VALU selects cost more than the LDS conflicts they save.
+__device__ __forceinline__ void hist_adds(double* wh, const uint32_t (&pk)[4], int d, double tmp2) {
+#pragma unroll
+  for (int j = 0; j < MV_FUSED_MAX_DIM; ++j)
+    if (j < d) lds_add(&wh[j * kBins + ((pk[j >> 2] >> (8 * (j & 3))) & 0xffu)], tmp2);
+}
+
 // One event, FP64 (vflow.py:87-126 + 33-83 + 389-430). Returns tmp = w f
 // and, when training, adds tmp^2 to the wave's histogram ``wh``.
 template <int FAM, bool TRAIN>
@@ -82,19 +92,7 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
     }
   }
   const double tmp = (w * wscale) * F.value();  // vflow.py:416
-  if (TRAIN) {
-    const double tmp2 = tmp * tmp;  // vflow.py:417
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      if (g < groups) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int j = 4 * g + q;
-          if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
-        }
-      }
-    }
-  }
+  if (TRAIN) hist_adds(wh, pk, d, tmp * tmp);  // vflow.py:417, 370-387
   return tmp;
 }
 
@@ -130,19 +128,7 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
   // w * f in FP32; the 1/N (and limits) factor in FP64 so that its rounding
   // is not a systematic bias
   const double tmp = (double)(w * F.value()) * wscale;
-  if (TRAIN) {
-    const double tmp2 = tmp * tmp;
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      if (g < groups) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int j = 4 * g + q;
-          if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
-        }
-      }
-    }
-  }
+  if (TRAIN) hist_adds(wh, pk, d, tmp * tmp);
   return tmp;
 }
 

@@ -191,18 +191,7 @@ __global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_in
       tmp2 = tmp * tmp;
       acc += tmp;
       acc2 += tmp2;
-      if (TRAIN) {
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          if (g < s.groups) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int j = 4 * g + q;
-              if (j < d) lds_add(&wh[j * kBins + ((pk[g] >> (8 * q)) & 0xffu)], tmp2);
-            }
-          }
-        }
-      }
+      if (TRAIN) hist_adds(wh, pk, d, tmp2);
     }
     segment_add(cube, tmp, tmp2, ress, ress2);
   }

@@ -115,6 +115,10 @@ def make_step(n_dim, n_events, *, precision="f64", train=True, event_offset=0,
     s.xmin = None if xmin is None else xmin.data_ptr()
     s.xdelta = None if xdelta is None else xdelta.data_ptr()
     s.inject = None if inject is None else inject.data_ptr()
+    # the struct holds raw addresses: keep the tensors alive with it, so that
+    # temporaries passed here are not freed (and their memory handed to a
+    # later allocation) before the launch that uses the struct
+    s._tensors = (xmin, xdelta, inject)
     return s
 
 
@@ -228,6 +232,7 @@ def make_strata(n_strat, n_cubes, n_ev, offsets):
     st.n_cubes = int(n_cubes)
     st.n_ev = n_ev.data_ptr()
     st.offsets = offsets.data_ptr()
+    st._tensors = (n_ev, offsets)  # see make_step
     return st
 
 
