@@ -210,6 +210,7 @@ def bench_serving(seconds, rate=200.0):
     """Poisson stream at ``rate`` req/s for ``seconds``: two-Gaussian mixture
     (d=4) or Breit-Wigner x power (d=6); target 1e-3 (1e6 events/iteration)
     or 1e-4 (2e7 events/iteration); at most 20 iterations."""
+    import numpy as np
     from vegasflow_amd.serve import Server, latency_report, poisson_requests
 
     reqs = poisson_requests(rate=rate, duration=seconds, seed=5)
@@ -218,9 +219,11 @@ def bench_serving(seconds, rate=200.0):
     done = server.run_stream(reqs)
     dt = time.perf_counter() - t0
     rep = latency_report(done)
+    ticks = np.array(server.tick_seconds) * 1e3
     rep.update({"rate_req_per_s": rate, "duration_s": seconds, "wall_s": dt,
                 "events_per_s": server.events_done / dt, "ticks": server.ticks,
-                "slots": server.capacity})
+                "tick_ms_p50": float(np.percentile(ticks, 50)), "tick_ms_p99": float(np.percentile(ticks, 99)),
+                "tick_ms_max": float(ticks.max()), "slots": server.capacity})
     return rep
 
 

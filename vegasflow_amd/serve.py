@@ -64,7 +64,7 @@ def slot_bytes():
 class Server:
     """Request loop over a slab of ``capacity`` slots (from memory_budget)."""
 
-    def __init__(self, memory_budget=64 << 20, dtype=None, min_iter=3, events_per_tile=4096,
+    def __init__(self, memory_budget=64 << 20, dtype=None, min_iter=3, events_per_tile=16384,
                  blocks_per_cu=4):
         kernels.require_gpu()
         self.device = default_device()
@@ -90,6 +90,8 @@ class Server:
         self.ticks = 0
         self.events_done = 0
         self._tile_cache = {}
+        self._launch = None  # device tiles / jobs / live slots, rebuilt when the live set changes
+        self.tick_seconds = []  # wall time of every tick
 
     # ---------------------------------------------------------- admission
     def submit(self, req):
@@ -119,14 +121,15 @@ class Server:
         req.slot = s
         req.admitted = now
         self.live[s] = req
+        self._launch = None
 
-    def _tiles_of(self, s, req):
-        key = (s, req.n_events)
+    def _tiles_of(self, s, req, ept):
+        key = (s, req.n_events, ept)
         t = self._tile_cache.get(key)
         if t is None:
             n = req.n_events
-            firsts = np.arange(0, n, self.events_per_tile, dtype=np.int64)
-            counts = np.minimum(self.events_per_tile, n - firsts).astype(np.int32)
+            firsts = np.arange(0, n, ept, dtype=np.int64)
+            counts = np.minimum(ept, n - firsts).astype(np.int32)
             t = np.zeros(len(firsts), dtype=[("item", "<i4"), ("count", "<i4"), ("first", "<i8")])
             t["item"] = s
             t["count"] = counts
@@ -135,37 +138,65 @@ class Server:
         return t
 
     # --------------------------------------------------------------- tick
+    def _build_launch(self):
+        """Device tiles per integrand family, finalize jobs and the live slot
+        list for the current live set (reused by every tick until a request
+        is admitted or retired)."""
+        by_family = collections.defaultdict(list)
+        jobs = []
+        slots = sorted(self.live)
+        for s in slots:
+            req = self.live[s]
+            by_family[req.integrand.family].append((s, req))
+            jobs.append((s, -1))
+            jobs.extend((s, j) for j in range(req.n_dim))
+        # tiles of at most events_per_tile events, smaller (powers of two down
+        # to 1024) when the live requests are too few to give every
+        # workgroup two tiles, so that a light load still spreads over the GPU
+        total = sum(self.live[s].n_events for s in slots)
+        ept = self.events_per_tile
+        while ept > 1024 and total < 2 * self.blocks * ept:
+            ept //= 2
+        if len(self._tile_cache) > 4 * self.capacity:
+            self._tile_cache.clear()
+        fams = []
+        for fam, members in sorted(by_family.items()):
+            tiles = np.concatenate([self._tiles_of(s, r, ept) for s, r in members])
+            tiles_dev = torch.from_numpy(tiles.view(np.uint8)).to(self.device)
+            fams.append((fam, tiles_dev, len(tiles), max(r.n_dim for _, r in members),
+                         sum(r.n_events for _, r in members)))
+        jobs_dev = torch.tensor(np.array(jobs, dtype=np.int32).reshape(-1), device=self.device)
+        slots_dev = torch.tensor(slots, dtype=torch.int64, device=self.device)
+        self._launch = (fams, jobs_dev, len(jobs), slots, slots_dev)
+
     def tick(self):
         """One iteration for every live slot; returns the requests retired."""
         if not self.live:
             return []
-        by_family = collections.defaultdict(list)
-        jobs = []
-        for s, req in self.live.items():
-            by_family[req.integrand.family].append((s, req))
-            jobs.append((s, -1))
-            jobs.extend((s, j) for j in range(req.n_dim))
-        tensors = []
-        for fam, members in sorted(by_family.items()):
-            tiles = np.concatenate([self._tiles_of(s, r) for s, r in members])
-            tiles_dev = torch.from_numpy(tiles.view(np.uint8)).to(self.device, non_blocking=False)
-            tensors.append(tiles_dev)
-            max_dim = max(r.n_dim for _, r in members)
-            kernels.batch_fused(self.items, self.capacity, tiles_dev, len(tiles), self.precision, fam, max_dim,
+        t_start = time.perf_counter()
+        if self._launch is None:
+            self._build_launch()
+        fams, jobs_dev, n_jobs, slots, slots_dev = self._launch
+        for fam, tiles_dev, n_tiles, max_dim, events in fams:
+            kernels.batch_fused(self.items, self.capacity, tiles_dev, n_tiles, self.precision, fam, max_dim,
                                 self.draws, self.grids, self.rows, self.blocks)
-            self.events_done += sum(r.n_events for _, r in members)
-        jobs_dev = torch.tensor(np.array(jobs, dtype=np.int32).reshape(-1), device=self.device)
-        kernels.batch_finalize(self.items, jobs_dev, len(jobs), self.grids, self.rows, self.draws, self.stats)
-        st = self.stats.view(-1, 4).cpu().numpy()  # synchronises
-        draws = self.draws.cpu().numpy()
+            self.events_done += events
+        kernels.batch_finalize(self.items, jobs_dev, n_jobs, self.grids, self.rows, self.draws, self.stats)
+        # the live slots' counters and running sums only (one small copy; synchronises)
+        live = torch.cat([self.stats.view(-1, 4).index_select(0, slots_dev),
+                          self.draws.index_select(0, slots_dev).to(torch.float64).unsqueeze(1)], 1)
+        rows = live.cpu().numpy()
         now = time.perf_counter()
         self.ticks += 1
+        self.tick_seconds.append(now - t_start)
         retired = []
-        for s, req in list(self.live.items()):
-            k = int(draws[s])
+        for r_i, s in enumerate(slots):
+            req = self.live[s]
+            st = rows[r_i]
+            k = int(st[4])
             with np.errstate(divide="ignore", invalid="ignore"):
-                est = st[s, 3] / st[s, 2]
-                err = np.sqrt(1.0 / st[s, 2])
+                est = st[3] / st[2]
+                err = np.sqrt(1.0 / st[2])
             req.iterations = k
             req.result, req.error = float(est), float(err)
             converged = k >= self.min_iter and np.isfinite(err) and err <= req.target * abs(est)
@@ -174,6 +205,8 @@ class Server:
                 retired.append(req)
                 del self.live[s]
                 self.free.append(s)
+        if retired:
+            self._launch = None
         self.finished.extend(retired)
         return retired
 
@@ -182,9 +215,29 @@ class Server:
             self._admit(self.queue.popleft(), now)
 
     # -------------------------------------------------------------- loops
-    def run_stream(self, requests, clock=time.perf_counter):
+    def warm_up(self, requests):
+        """Load the code objects the stream will use (one tiny iteration per
+        integrand family and dimension present) before serving, so that no
+        request waits for a first-launch load. Leaves the server empty."""
+        seen = {}
+        for r in requests:
+            seen.setdefault((r.integrand.family, r.n_dim), r)
+        for (_, d), r in seen.items():
+            self.submit(Request(r.integrand, d, 64, target=1.0, max_iter=1, seed=1))
+        self.admit_waiting(float("inf"))
+        while self.live:
+            self.tick()
+        torch.cuda.synchronize()
+        self.finished = []
+        self.ticks = 0
+        self.events_done = 0
+        self.tick_seconds = []
+
+    def run_stream(self, requests, clock=time.perf_counter, warm_up=True):
         """Serve ``requests`` (sorted by ``arrival``, seconds from the
         start) in real time; returns the finished requests."""
+        if warm_up:
+            self.warm_up(requests)
         t0 = clock()
         for r in requests:
             r.arrival = t0 + r.arrival
