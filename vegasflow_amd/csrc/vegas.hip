@@ -86,7 +86,10 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand 
 }
 
 template <int FAM, bool TRAIN>
-__global__ __launch_bounds__(kBlock) void vegas_fused_mixed(StepK s, mv_integrand fn,
+// Eight waves per SIMD (the compiler keeps the SGPR count low enough): the
+// training kernel is bound by the LDS histogram atomics, and more resident
+// waves overlap them better with the Philox VALU work.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void vegas_fused_mixed(StepK s, mv_integrand fn,
                                                             const double* __restrict__ grid,
                                                             RowsK R) {
   extern __shared__ double smem[];
