@@ -400,11 +400,13 @@ def test_vegas_sample_mixed_matches_oracle_positions(d):
     assert x.min().item() > 0.0 and x.max().item() <= 1.0
 
 
-def test_mixed_fused_matches_oracle_positions():
+@pytest.mark.parametrize("d", [1, 6, 9, 14, 16])
+def test_mixed_fused_matches_oracle_positions(d):
     """The fused mixed iteration equals the FP64 oracle evaluated on the
     mixed positions (same words) to FP32 accuracy; bins are exact, so the
-    histogram counts agree."""
-    d, n = 6, 300000
+    histogram counts agree. d = 6, 9, 14 train through pair histograms,
+    d = 1 and 16 through per-dimension ones."""
+    n = 300000
     rng = np.random.default_rng(12)
     grid = random_grid(rng, d)
     out, hist, _ = _fused_once("gauss_offc", d, n, grid, True, seed=31, precision="mixed")
@@ -429,10 +431,13 @@ def test_mixed_fused_tracks_f64_on_same_draws():
     assert rel(h32.sum(), h64.sum()) < 1e-4
 
 
-def test_mixed_fused_bins_match_integer_oracle():
+@pytest.mark.parametrize("d", [2, 3, 11, 15])
+def test_mixed_fused_bins_match_integer_oracle(d):
     """Mixed bins are those of oracle.mixed_position: compare the histogram
-    of a constant integrand (counts per bin) with the oracle's integer bins."""
-    d, n = 3, 200000
+    of a constant integrand (counts per bin) with the oracle's integer bins
+    (pair-histogram marginals for d <= 14, odd d with an unpaired last
+    dimension included)."""
+    n = 200000
     out, hist, _ = _fused_once("const", d, n, ora.initial_grid(d), True, seed=4, precision="mixed")
     bins, _ = ora.mixed_position(ora.hi_words(4, 0, n, d))
     tmp2 = (2.5 / n) ** 2

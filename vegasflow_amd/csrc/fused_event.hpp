@@ -57,9 +57,54 @@ __device__ __forceinline__ void hist_adds(double* wh, const uint32_t (&pk)[4], i
     if (j < d) lds_add(&wh[j * kBins + ((pk[j >> 2] >> (8 * (j & 3))) & 0xffu)], tmp2);
 }
 
+// Pair histograms: dimensions 2p and 2p + 1 share one 50 x 50 table of
+// cells, so an event makes one LDS add per pair instead of one per
+// dimension, and the adds of a wave spread over 2500 cells instead of 50,
+// which removes most same-address conflicts of the LDS atomics. The
+// per-dimension histograms are the row and column sums (pair_marginal).
+// LDS: ceil(d / 2) * 2500 doubles, so only for d <= kPairMaxDim.
+constexpr int kPairBins = kBins * kBins;
+constexpr int kPairMinDim = 2;
+constexpr int kPairMaxDim = 14;
+
+__host__ __device__ constexpr int pair_cells(int d) { return ((d + 1) >> 1) * kPairBins; }
+
+// cell (b_2p, b_2p+1) of pair p gets tmp2; b_2p+1 = 0 (its byte of pk is
+// never set) for the unpaired last dimension of an odd d
+__device__ __forceinline__ void pair_adds(double* H, const uint32_t (&pk)[4], int d, double tmp2) {
+#pragma unroll
+  for (int p = 0; p < kPairMaxDim / 2; ++p) {
+    if (2 * p < d) {
+      const uint32_t w = pk[p >> 1] >> (16 * (p & 1));
+      lds_add(&H[p * kPairBins + (w & 0xffu) * kBins + ((w >> 8) & 0xffu)], tmp2);
+    }
+  }
+}
+
+// Bin b of dimension j = m / 50 (b = m % 50) from the pair histograms: row
+// b of pair j / 2 for an even j, column b for an odd j, summed in order.
+__device__ __forceinline__ double pair_marginal(const double* H, int m) {
+  const int j = m / kBins, b = m - j * kBins;
+  const double* c = H + (j >> 1) * kPairBins + ((j & 1) ? b : b * kBins);
+  const int stride = (j & 1) ? kBins : 1;
+  double a = 0.0;
+#pragma unroll 10
+  for (int k = 0; k < kBins; ++k) a += c[k * stride];
+  return a;
+}
+
+template <bool PAIRS>
+__device__ __forceinline__ void event_hist(double* wh, const uint32_t (&pk)[4], int d, double tmp2) {
+  if (PAIRS)
+    pair_adds(wh, pk, d, tmp2);
+  else
+    hist_adds(wh, pk, d, tmp2);
+}
+
 // One event, FP64 (vflow.py:87-126 + 33-83 + 389-430). Returns tmp = w f
-// and, when training, adds tmp^2 to the wave's histogram ``wh``.
-template <int FAM, bool TRAIN>
+// and, when training, adds tmp^2 to the histogram ``wh`` (the wave's
+// per-dimension histograms, or the workgroup's pair histograms if PAIRS).
+template <int FAM, bool TRAIN, bool PAIRS = false>
 __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t c0, uint32_t c1,
                                             uint32_t draw, Key key, int groups, int d,
                                             const double2* tab, const double* tw, double* wh,
@@ -92,13 +137,13 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
     }
   }
   const double tmp = (w * wscale) * F.value();  // vflow.py:416
-  if (TRAIN) hist_adds(wh, pk, d, tmp * tmp);  // vflow.py:417, 370-387
+  if (TRAIN) event_hist<PAIRS>(wh, pk, d, tmp * tmp);  // vflow.py:417, 370-387
   return tmp;
 }
 
 // One event, mixed precision: FP32 map and integrand from the high word of
 // each draw (mixed_position), FP64 result.
-template <int FAM, bool TRAIN>
+template <int FAM, bool TRAIN, bool PAIRS = false>
 __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
                                               uint32_t draw, Key key, int groups, int d,
                                               const float2* tab, const float* cj, double* wh,
@@ -128,7 +173,7 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
   // w * f in FP32; the 1/N (and limits) factor in FP64 so that its rounding
   // is not a systematic bias
   const double tmp = (double)(w * F.value()) * wscale;
-  if (TRAIN) hist_adds(wh, pk, d, tmp * tmp);
+  if (TRAIN) event_hist<PAIRS>(wh, pk, d, tmp * tmp);
   return tmp;
 }
 

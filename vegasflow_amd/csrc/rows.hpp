@@ -14,6 +14,7 @@
 
 #include "abi_util.hpp"
 #include "device.hpp"
+#include "fused_event.hpp"
 
 namespace mv {
 
@@ -86,8 +87,8 @@ struct RowsK {
   int64_t stride;
 };
 
-inline int producer_blocks(int64_t n_events) {
-  int64_t b = (n_events + kBlock - 1) / kBlock;
+inline int producer_blocks(int64_t n_events, int block = kBlock) {
+  int64_t b = (n_events + block - 1) / block;
   if (b < 1) b = 1;
   if (b > MV_MAX_BLOCKS) b = MV_MAX_BLOCKS;
   return (int)b;
@@ -95,22 +96,26 @@ inline int producer_blocks(int64_t n_events) {
 
 inline int rows_used(int blocks) { return blocks < kRows ? blocks : kRows; }
 
-// Workgroups of ``kernel`` (kBlock threads, ``lds`` dynamic LDS bytes) that
-// are resident on the current device at once: occupancy per CU x CUs, cached
-// per (device, kernel, lds). 0 if the runtime cannot tell.
-int resident_blocks(const void* kernel, size_t lds);
+// Workgroups of ``kernel`` (``block`` threads, ``lds`` dynamic LDS bytes)
+// that are resident on the current device at once: occupancy per CU x CUs,
+// cached per (device, kernel, block, lds). 0 if the runtime cannot tell or
+// the workgroup does not fit.
+int resident_blocks(const void* kernel, size_t lds, int block = kBlock);
+
+// Dynamic LDS bytes one workgroup may use on the current device (cached).
+size_t max_lds_per_block();
 
 // Launch a grid-stride producer over ``n`` events with producer_blocks(n)
 // workgroups capped at one device-wide round of resident workgroups: every
 // workgroup owns an equal share of the events, so a grid larger than what
 // fits at once would leave a second, mostly idle round at the end.
 // Returns the number of workgroups launched.
-template <typename... KArgs, typename... Args>
+template <int BS = kBlock, typename... KArgs, typename... Args>
 int launch_producer(void (*kernel)(KArgs...), int64_t n, size_t lds, hipStream_t st, Args... args) {
-  int blocks = producer_blocks(n);
-  const int fit = resident_blocks(reinterpret_cast<const void*>(kernel), lds);
+  int blocks = producer_blocks(n, BS);
+  const int fit = resident_blocks(reinterpret_cast<const void*>(kernel), lds, BS);
   if (fit > 0 && blocks > fit) blocks = fit;
-  kernel<<<dim3(blocks), dim3(kBlock), lds, st>>>(args...);
+  kernel<<<dim3(blocks), dim3(BS), lds, st>>>(args...);
   return blocks;
 }
 
@@ -161,6 +166,33 @@ __device__ __forceinline__ void finish_row(double acc, double acc2, const double
       for (int k = 1; k < kWaves; ++k) h += hist[k * hist_len + i];
       if (h != 0.0) row_add(&row[2 + i], h);
     }
+  }
+}
+
+// Workgroup end of the pair-histogram producers (BS threads): sums as in
+// finish_row, histogram bins from the pair tables (pair_marginal).
+template <int BS>
+__device__ __forceinline__ void finish_pairs(double acc, double acc2, const double* H, int d,
+                                             double* row) {
+  constexpr int W = BS / 64;
+  __shared__ double red[2 * W];
+  acc = wave_sum(acc);
+  acc2 = wave_sum(acc2);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wave] = acc;
+    red[W + wave] = acc2;
+  }
+  __syncthreads();  // also: every histogram add has landed
+  if (threadIdx.x < 2) {
+    const double* r = red + threadIdx.x * W;
+    double a = r[0];
+    for (int k = 1; k < W; ++k) a += r[k];
+    row_add(&row[threadIdx.x], a);
+  }
+  for (int m = threadIdx.x; m < d * kBins; m += BS) {
+    const double h = pair_marginal(H, m);
+    if (h != 0.0) row_add(&row[2 + m], h);
   }
 }
 

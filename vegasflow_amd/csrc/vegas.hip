@@ -28,12 +28,12 @@ std::string& last_error_slot() {
   return s;
 }
 
-int resident_blocks(const void* kernel, size_t lds) {
+int resident_blocks(const void* kernel, size_t lds, int block) {
   static std::mutex mu;
-  static std::map<std::tuple<int, const void*, size_t>, int> cache;
+  static std::map<std::tuple<int, const void*, int, size_t>, int> cache;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
-  const auto key = std::make_tuple(dev, kernel, lds);
+  const auto key = std::make_tuple(dev, kernel, block, lds);
   {
     std::lock_guard<std::mutex> g(mu);
     auto it = cache.find(key);
@@ -41,7 +41,7 @@ int resident_blocks(const void* kernel, size_t lds) {
   }
   int cus = 0, per_cu = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds) != hipSuccess) {
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess) {
     (void)hipGetLastError();
     return 0;
   }
@@ -49,6 +49,23 @@ int resident_blocks(const void* kernel, size_t lds) {
   std::lock_guard<std::mutex> g(mu);
   cache[key] = r;
   return r;
+}
+
+size_t max_lds_per_block() {
+  static std::mutex mu;
+  static std::map<int, size_t> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    v = 0;
+  }
+  cache[dev] = (size_t)v;
+  return (size_t)v;
 }
 
 // ---------------------------------------------------------------------------
@@ -118,6 +135,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     acc2 += tmp * tmp;
   }
   finish_block<TRAIN>(acc, acc2, hist, nb, R);
+}
+
+// Mixed-precision training variant with pair histograms (fused_event.hpp)
+// for n_dim <= kPairMaxDim: one LDS add per pair of dimensions, which halves
+// the histogram atomics that bound vegas_fused_mixed. BS-thread workgroups
+// share one set of pair tables; LDS: the map table (nb float2), then
+// pair_cells(d) doubles, about 83 KB at d = 8, so one workgroup per CU. (The
+// FP64 kernel is bound by its VALU work and measured slower at that
+// occupancy, so it keeps per-wave histograms.)
+template <int FAM, int BS>
+__global__ __launch_bounds__(BS) void vegas_fused_pairs(StepK s, mv_integrand fn,
+                                                        const double* __restrict__ grid, RowsK R) {
+  extern __shared__ double smem[];
+  const int d = s.n_dim;
+  const int nb = d * kBins;
+  float2* tab = reinterpret_cast<float2*>(smem);
+  double* H = smem + nb;  // 16-byte aligned: nb * 8 is a multiple of 400
+  build_table<float, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, nullptr);
+  const int half = pair_cells(d) / 2;
+  for (int i = threadIdx.x; i < half; i += BS) reinterpret_cast<double2*>(H)[i] = make_double2(0.0, 0.0);
+  float cj[MV_FUSED_MAX_DIM];
+#pragma unroll
+  for (int j = 0; j < MV_FUSED_MAX_DIM; ++j) cj[j] = j < d ? jacobian_scale<FAM>(fn, j, s.xdelta) : 0.0f;
+  __syncthreads();
+  Integrand<float, FAM> F(fn);
+  const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
+  const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
+  double acc = 0.0, acc2 = 0.0;
+  const int64_t step = (int64_t)gridDim.x * BS;
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < s.n_events; i += step) {
+    const double tmp = event_mixed<FAM, true, true>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id,
+                                                    s.key, s.groups, d, tab, cj, H,
+                                                    inj ? inj + i * d : nullptr, wscale);
+    acc += tmp;
+    acc2 += tmp * tmp;
+  }
+  finish_pairs<BS>(acc, acc2, H, d, group_row(R));
 }
 
 // ---------------------------------------------------------------------------
@@ -517,11 +571,25 @@ extern "C" int32_t mv_producer_rows(int64_t n_events) {
 
 namespace {
 
+constexpr int kPairBlock = 1024;  // threads of a pair-histogram workgroup
+
+// LDS bytes of vegas_fused_pairs, 0 where it does not apply (FP64, no
+// training, n_dim outside kPairMinDim..kPairMaxDim, or more LDS than a
+// workgroup may have)
+size_t pairs_lds(int d, bool mixed, bool train) {
+  if (!mixed || !train || d < kPairMinDim || d > kPairMaxDim) return 0;
+  const size_t lds = (size_t)(d * kBins + pair_cells(d)) * sizeof(double);
+  return lds <= max_lds_per_block() ? lds : 0;
+}
+
 template <int FAM>
 int launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool train,
                        const double* grid, const RowsK& R, hipStream_t st) {
   const int nb = k.n_dim * kBins;
   const int64_t n = k.n_events;
+  if (const size_t plds = pairs_lds(k.n_dim, mixed, train))
+    return launch_producer<kPairBlock>(vegas_fused_pairs<FAM, kPairBlock>, n, plds, st, k, fn,
+                                       grid, R);
   if (!mixed) {
     const size_t lds = (size_t)(3 * nb + (train ? kWaves * nb : 0)) * sizeof(double);
     return train ? launch_producer(vegas_fused_f64<FAM, true>, n, lds, st, k, fn, grid, R)
