@@ -23,7 +23,7 @@ namespace mv {
 template <bool MIXED, int FAM, bool TRAIN>
 __device__ __forceinline__ void batch_events(const mv_batch_item& it, const mv_batch_tile& tile,
                                              uint32_t draw, const void* tab, const void* tw,
-                                             const float* cj, double* hist, int nb, double* row) {
+                                             float cprod, double* hist, int nb, double* row) {
   using T = typename std::conditional<MIXED, float, double>::type;
   Integrand<T, FAM> F(it.fn);
   const Key key{(uint32_t)(it.seed & 0xffffffffu), (uint32_t)(it.seed >> 32)};
@@ -40,7 +40,7 @@ __device__ __forceinline__ void batch_events(const mv_batch_item& it, const mv_b
     double tmp;
     if constexpr (MIXED)
       tmp = event_mixed<FAM, TRAIN>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
-                                    static_cast<const float2*>(tab), cj, wh, nullptr, wscale);
+                                    static_cast<const float2*>(tab), cprod, wh, nullptr, wscale);
     else
       tmp = event_f64<FAM, TRAIN>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
                                   static_cast<const double2*>(tab), static_cast<const double*>(tw), wh,
@@ -67,16 +67,13 @@ __device__ __forceinline__ void batch_tile(const mv_batch_item& it, const mv_bat
   build_table<T, FAM>(it.fn, grids + it.grid_offset, d, nullptr, nullptr, tab, tw);
   if (it.train)
     for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
-  float cj[MV_FUSED_MAX_DIM];
-#pragma unroll
-  for (int j = 0; j < MV_FUSED_MAX_DIM; ++j)
-    cj[j] = MIXED && j < d ? jacobian_scale<FAM>(it.fn, j, nullptr) : 0.0f;
+  const float cprod = MIXED ? jacobian_product<FAM>(it.fn, d, nullptr) : 0.0f;
   __syncthreads();
   double* row = rows + it.row_offset;
   if (it.train)
-    batch_events<MIXED, FAM, true>(it, tile, draw, tab, tw, cj, hist, nb, row);
+    batch_events<MIXED, FAM, true>(it, tile, draw, tab, tw, cprod, hist, nb, row);
   else
-    batch_events<MIXED, FAM, false>(it, tile, draw, tab, tw, cj, hist, nb, row);
+    batch_events<MIXED, FAM, false>(it, tile, draw, tab, tw, cprod, hist, nb, row);
 }
 
 // FAM >= 0: every tile belongs to an item of that family (smaller register

@@ -6,8 +6,9 @@
 // and, in FP64, the Jacobian factor W = dx * 50 (vflow.py:73, 78). The FP32
 // table of the mixed path takes 1 + frac (mixed_position), so it stores
 // (P - Q, Q), and has no W column: W = Q * c_j with the per-dimension
-// constant c_j = 50 / (xdelta_j * fold scale_j) (jacobian_scale), one
-// multiply in registers instead of one more LDS read per dimension.
+// constant c_j = 50 / (xdelta_j * fold scale_j) (jacobian_scale), so an
+// event multiplies its slopes Q_j and then the product of the c_j once,
+// instead of one more LDS read per dimension.
 #pragma once
 
 #include <type_traits>
@@ -45,6 +46,15 @@ template <int FAM>
 __device__ __forceinline__ float jacobian_scale(const mv_integrand& fn, int j, const double* xdelta) {
   const double xd = xdelta ? xdelta[j] : 1.0;
   return (float)(kFBins / (xd * Integrand<float, FAM>::fold_scale(fn, j)));
+}
+
+// prod_j c_j over the n_dim dimensions, in order: the mixed kernels
+// multiply the slopes Q_j of an event and apply this product once.
+template <int FAM>
+__device__ __forceinline__ float jacobian_product(const mv_integrand& fn, int d, const double* xdelta) {
+  float c = 1.0f;
+  for (int j = 0; j < d; ++j) c = c * jacobian_scale<FAM>(fn, j, xdelta);
+  return c;
 }
 
 // Histogram adds of one event: tmp2 into bin pk[j] of every dimension j
@@ -146,7 +156,7 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
 template <int FAM, bool TRAIN, bool PAIRS = false>
 __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
                                               uint32_t draw, Key key, int groups, int d,
-                                              const float2* tab, const float* cj, double* wh,
+                                              const float2* tab, float cprod, double* wh,
                                               const uint32_t* inj, double wscale) {
   F.begin();
   float w = 1.0f;
@@ -164,7 +174,7 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
           const int ind = mixed_position(inj ? inj[j] : word(h, q), f1);
           const float2 t = tab[j * kBins + ind];
           F.add_folded(j, fmaf(t.y, f1, t.x));
-          w = w * (t.y * cj[j]);
+          w = w * t.y;
           if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
         }
       }
@@ -172,8 +182,82 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
   }
   // w * f in FP32; the 1/N (and limits) factor in FP64 so that its rounding
   // is not a systematic bias
-  const double tmp = (double)(w * F.value()) * wscale;
+  const double tmp = (double)((w * cprod) * F.value()) * wscale;
   if (TRAIN) event_hist<PAIRS>(wh, pk, d, tmp * tmp);
+  return tmp;
+}
+
+// One dimension of a mixed event: position of draw word hw in dimension j,
+// the integrand coordinate into F, the map slope into w; returns the bin.
+template <int FAM>
+__device__ __forceinline__ uint32_t mixed_dim(Integrand<float, FAM>& F, int j, uint32_t hw,
+                                              const float2* tab, float& w) {
+  float f1;
+  const int b = mixed_position(hw, f1);
+  const float2 t = tab[j * kBins + b];
+  F.add_folded(j, fmaf(t.y, f1, t.x));
+  w = w * t.y;
+  return (uint32_t)b;
+}
+
+template <int FAM>
+__device__ __forceinline__ void pair_dim(Integrand<float, FAM>& F, int j, uint32_t hw,
+                                         const float2* tab, float& w,
+                                         uint32_t (&cell)[kPairMaxDim / 2]) {
+  const uint32_t b = mixed_dim<FAM>(F, j, hw, tab, w);
+  if (j & 1)
+    cell[j >> 1] += b * (uint32_t)sizeof(double);
+  else
+    cell[j >> 1] = b * (uint32_t)(kBins * sizeof(double));
+}
+
+// One training event of the pair-histogram mixed kernel (event_mixed with
+// PAIRS, restructured for fewer instructions): the byte offset of the event's
+// cell in each pair table is built while the dimensions are mapped (no bin
+// packing), groups of four dimensions that are all present skip the
+// per-dimension checks, and INJ (injected words instead of Philox) is a
+// template argument rather than a test per dimension. Adds tmp^2 to the pair
+// tables at H and returns tmp.
+template <int FAM, bool INJ>
+__device__ __forceinline__ double event_mixed_pairs(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
+                                                    uint32_t draw, Key key, int d, const float2* tab,
+                                                    float cprod, double* H, const uint32_t* inj,
+                                                    double wscale) {
+  F.begin();
+  float w = 1.0f;
+  uint32_t cell[kPairMaxDim / 2];
+#pragma unroll
+  for (int p = 0; p < kPairMaxDim / 2; ++p) cell[p] = 0u;
+#pragma unroll
+  for (int g = 0; g < (kPairMaxDim + 3) / 4; ++g) {
+    if (4 * g < d) {
+      const bool full = 4 * g + 4 <= d;
+      uint4 h;
+      if (INJ) {
+        h.x = inj[4 * g];
+        h.y = full || 4 * g + 1 < d ? inj[4 * g + 1] : 0u;
+        h.z = full || 4 * g + 2 < d ? inj[4 * g + 2] : 0u;
+        h.w = full ? inj[4 * g + 3] : 0u;
+      } else {
+        h = philox(c0, c1, draw, (uint32_t)g, key);
+      }
+      if (full) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (4 * g + q < kPairMaxDim) pair_dim<FAM>(F, 4 * g + q, word(h, q), tab, w, cell);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          if (4 * g + q < kPairMaxDim && 4 * g + q < d) pair_dim<FAM>(F, 4 * g + q, word(h, q), tab, w, cell);
+      }
+    }
+  }
+  const double tmp = (double)((w * cprod) * F.value()) * wscale;
+  const double tmp2 = tmp * tmp;
+  char* Hb = reinterpret_cast<char*>(H);
+#pragma unroll
+  for (int p = 0; p < kPairMaxDim / 2; ++p)
+    if (2 * p < d) lds_add(reinterpret_cast<double*>(Hb + p * kPairBins * sizeof(double) + cell[p]), tmp2);
   return tmp;
 }
 

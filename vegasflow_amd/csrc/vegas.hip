@@ -117,9 +117,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   build_table<float, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, nullptr);
   if (TRAIN)
     for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
-  float cj[MV_FUSED_MAX_DIM];
-#pragma unroll
-  for (int j = 0; j < MV_FUSED_MAX_DIM; ++j) cj[j] = j < d ? jacobian_scale<FAM>(fn, j, s.xdelta) : 0.0f;
+  const float cprod = jacobian_product<FAM>(fn, d, s.xdelta);
   __syncthreads();
   double* wh = hist + (threadIdx.x >> 6) * nb;
   Integrand<float, FAM> F(fn);
@@ -129,7 +127,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
     const double tmp = event_mixed<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id, s.key,
-                                               s.groups, d, tab, cj, wh, inj ? inj + i * d : nullptr,
+                                               s.groups, d, tab, cprod, wh, inj ? inj + i * d : nullptr,
                                                wscale);
     acc += tmp;
     acc2 += tmp * tmp;
@@ -155,21 +153,28 @@ __global__ __launch_bounds__(BS) void vegas_fused_pairs(StepK s, mv_integrand fn
   build_table<float, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, nullptr);
   const int half = pair_cells(d) / 2;
   for (int i = threadIdx.x; i < half; i += BS) reinterpret_cast<double2*>(H)[i] = make_double2(0.0, 0.0);
-  float cj[MV_FUSED_MAX_DIM];
-#pragma unroll
-  for (int j = 0; j < MV_FUSED_MAX_DIM; ++j) cj[j] = j < d ? jacobian_scale<FAM>(fn, j, s.xdelta) : 0.0f;
+  const float cprod = jacobian_product<FAM>(fn, d, s.xdelta);
   __syncthreads();
   Integrand<float, FAM> F(fn);
   const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
   const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * BS;
-  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < s.n_events; i += step) {
-    const double tmp = event_mixed<FAM, true, true>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id,
-                                                    s.key, s.groups, d, tab, cj, H,
-                                                    inj ? inj + i * d : nullptr, wscale);
-    acc += tmp;
-    acc2 += tmp * tmp;
+  const int64_t i0 = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (inj) {
+    for (int64_t i = i0; i < s.n_events; i += step) {
+      const double tmp = event_mixed_pairs<FAM, true>(F, 0u, 0u, 0u, s.key, d, tab, cprod, H,
+                                                      inj + i * d, wscale);
+      acc += tmp;
+      acc2 += tmp * tmp;
+    }
+  } else {
+    for (int64_t i = i0; i < s.n_events; i += step) {
+      const double tmp = event_mixed_pairs<FAM, false>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id,
+                                                       s.key, d, tab, cprod, H, nullptr, wscale);
+      acc += tmp;
+      acc2 += tmp * tmp;
+    }
   }
   finish_pairs<BS>(acc, acc2, H, d, group_row(R));
 }
