@@ -24,7 +24,9 @@ constexpr int kMaxRows = 2048;                     // partial rows per producer 
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11). The key schedule depends only on the
-// seed, which is wave-uniform, so it stays in scalar registers.
+// seed, which is wave-uniform, so it stays in scalar registers. Each round's
+// two three-way XORs are one v_bitop3_b32 each (truth table 0x96 = a^b^c,
+// new in gfx950) instead of two v_xor_b32.
 // ---------------------------------------------------------------------------
 struct Key {
   uint32_t k0, k1;
@@ -39,9 +41,9 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
     const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    c0 = hi1 ^ c1 ^ k0;
+    c0 = __builtin_amdgcn_bitop3_b32(hi1, c1, k0, 0x96);  // hi1 ^ c1 ^ k0
     c1 = lo1;
-    c2 = hi0 ^ c3 ^ k1;
+    c2 = __builtin_amdgcn_bitop3_b32(hi0, c3, k1, 0x96);  // hi0 ^ c3 ^ k1
     c3 = lo0;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
