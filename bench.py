@@ -34,8 +34,13 @@ PEAK = {"mixed": 157.3, "f64": 78.6}  # TFLOP/s, MI355X datasheet (FP32 / FP64 v
 # FETCH_SIZE and WRITE_SIZE passes of rocprofv3 scaled by a 1 GiB copy
 # (profiles/vegas_fused_gauss_d8_1e7_pmc.md): 0 bytes per event; the grid
 # reads and the FP64 atomic adds of the partial rows
-TRAFFIC = {"mixed": 6.146e6, "f64": 5.314e6}
+TRAFFIC = {"mixed": 1.0025e6, "f64": 5.3197e6}
 TRAFFIC_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_pmc.md"
+# VALU wave instructions per launch (SQ_INSTS_VALU, same profile) and the
+# issue peak they are priced against: one wave64 VALU instruction per SIMD
+# every 4 cycles, 4 SIMDs per CU, 2.4 GHz engine clock (MI355X datasheet)
+VALU_INSTS = {"mixed": 3.099e7, "f64": 6.379e7}
+CLOCK_HZ = 2.4e9
 
 
 def flops_per_event(d, train=True):
@@ -386,6 +391,14 @@ def main():
     head = results["mixed"]
     flop = flops_per_event(D, True) * N
     achieved = flop / (head["fused_kernel_ms"] * 1e-3) / 1e12
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    valu_peak = cus * 4 * CLOCK_HZ / 4  # wave instructions per second
+
+    def valu_issue(prec):
+        rate = VALU_INSTS[prec] / (results[prec]["fused_kernel_ms"] * 1e-3)
+        return {"insts_per_launch": VALU_INSTS[prec], "achieved_G_per_s": rate / 1e9,
+                "peak_G_per_s": valu_peak / 1e9, "frac": rate / valu_peak}
+
     line = {
         "metric": "VEGAS events/s, d=8 Gaussian, mixed vs FP64 at equal error",
         "value": head["events_per_s"],
@@ -407,7 +420,7 @@ def main():
         "errors": err,
         "roofline": {
             "bound": "valu",
-            "kernel": "vegas_fused_mixed<GAUSSIAN, train>",
+            "kernel": "vegas_fused_pairs<GAUSSIAN, 1024> (mixed training)",
             "achieved": achieved,
             "peak": PEAK["mixed"],
             "unit": "TFLOP/s",
@@ -417,10 +430,13 @@ def main():
             "traffic_source": TRAFFIC_SOURCE,
             "kernel_ms": head["fused_kernel_ms"],
             "flop_per_event": flops_per_event(D, True),
+            "valu_issue": valu_issue("mixed"),
             "f64_kernel": {
+                "kernel": "vegas_fused_f64<GAUSSIAN, train>",
                 "achieved": flop / (results["f64"]["fused_kernel_ms"] * 1e-3) / 1e12,
                 "peak": PEAK["f64"],
                 "traffic": TRAFFIC["f64"],
+                "valu_issue": valu_issue("f64"),
             },
         },
     }
