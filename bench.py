@@ -205,7 +205,8 @@ def bench_batched(torch, dtype, n_iter=8):
     pulls = (res - exact) / err
     return {"events_per_s": b.events_per_round() * n_iter / dt, "ms_per_round": 1e3 * dt / n_iter,
             "events_per_round": b.events_per_round(), "items": len(specs),
-            "events_per_tile": b.events_per_tile, "workgroups": b.blocks,
+            "events_per_tile": b.events_per_tile, "pair_histograms": b.pairs,
+            "workgroups": max(b._pair_blocks.values()) if b.pairs else b.blocks,
             "frac_within_3sigma": float(np.mean(np.abs(pulls) < 3)),
             "median_abs_pull": float(np.median(np.abs(pulls))),
             "median_chi2_dof": float(np.nanmedian(b.chi2_dof()))}

@@ -278,6 +278,18 @@ int mv_batch_fused(const mv_batch_item* items, int32_t n_items, const mv_batch_t
                    const int32_t* draws, const double* grids, double* rows, int32_t blocks,
                    void* stream);
 
+/* As mv_batch_fused in MV_MIXED, with the histograms of training items kept
+ * as pair tables (dimensions 2p and 2p + 1 share a 50 x 50 table: one LDS
+ * atomic per pair of dimensions) in 1024-thread workgroups; frozen items
+ * run as in mv_batch_fused. Requires max_dim <= 14. ``blocks`` persistent
+ * workgroups of 1024 threads; mv_batch_pairs_blocks(family, max_dim) is the
+ * number resident on the current device at once (0: unsupported). No
+ * counterpart in the reference (batching is an addition). */
+int mv_batch_fused_pairs(const mv_batch_item* items, int32_t n_items, const mv_batch_tile* tiles,
+                         int32_t n_tiles, int32_t family, int32_t max_dim, const int32_t* draws,
+                         const double* grids, double* rows, int32_t blocks, void* stream);
+int32_t mv_batch_pairs_blocks(int32_t family, int32_t max_dim);
+
 /* Iteration end for the items named in ``jobs`` (device int32 pairs
  * {item, dim}: dim = -1 reduces the item's sums into stats (vflow.py:437-438,
  * monte_carlo.py:718-720), stores {integral, sigma} at history[2 (item *

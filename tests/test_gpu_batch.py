@@ -47,6 +47,50 @@ def test_batch_items_match_lone_runs(dtype):
     np.testing.assert_array_equal(b.iterations(), 3)
 
 
+def pair_specs():
+    return [
+        (vf.Gaussian(0.2, [0.3, 0.5, 0.6]), 3, 20000, 101),
+        (vf.Product(), 2, 5000, 102),
+        (vf.GaussianMix(), 4, 30011, 103),
+        (vf.BreitWignerPower(), 6, 10000, 104),
+        (vf.Constant(3.0), 1, 1000, 105),
+        (vf.Gaussian(0.1, 0.5), 12, 70000, 106),
+        (vf.Gaussian(0.15, 0.45), 13, 9000, 107),
+        (vf.Gaussian(0.12, 0.55), 14, 40000, 108),
+    ]
+
+
+def test_batch_pair_histograms_match_lone_runs():
+    """Mixed batch through mv_batch_fused_pairs (tiles of 16384 events over
+    1024-thread workgroups), replayed from a graph: every item equals its lone
+    run (odd n_dim, n_dim = 1 and 14 included)."""
+    specs = pair_specs()
+    b = BatchVegas(specs, dtype=torch.float32, events_per_tile=16384, pair_histograms=True)
+    assert b.pairs and all(v > 0 for v in b._pair_blocks.values())
+    b.run(3, graph=True)
+    hist = b.history()
+    for i, (fn, d, n, seed) in enumerate(specs):
+        lone = vf.VegasFlow(d, n, seed=seed, verbose=False, dtype=torch.float32)
+        lone.compile(fn)
+        lone.run_integration(3)
+        for k, (r, s, _) in enumerate(lone.history):
+            assert abs(hist[i, k, 0] - r) <= 1e-10 * abs(r) + 1e-300, (i, k)
+            assert abs(hist[i, k, 1] - s) <= 1e-7 * abs(s) + 1e-300, (i, k)
+        np.testing.assert_allclose(b.grid(i).cpu().numpy(), lone.divisions.cpu().numpy(),
+                                   rtol=1e-10, atol=1e-13)
+
+
+def test_batch_pair_kernel_frozen_items_match_per_wave_kernel():
+    """Items that do not train run the plain mixed event body inside the pair
+    kernel: same sums as the per-wave kernel on the same draws."""
+    specs = pair_specs()
+    a = BatchVegas(specs, dtype=torch.float32, train=False, events_per_tile=4096, blocks_per_cu=4)
+    b = BatchVegas(specs, dtype=torch.float32, train=False, events_per_tile=65536, pair_histograms=True)
+    a.run(2, graph=False)
+    b.run(2, graph=False)
+    np.testing.assert_allclose(b.history(), a.history(), rtol=1e-10)
+
+
 def test_batch_graph_replays_match_eager_rounds():
     specs = small_specs()
     eager = BatchVegas(specs, events_per_tile=4096, blocks_per_cu=4)
@@ -73,11 +117,14 @@ def test_autotune_picks_and_caches_a_shape():
     try:
         specs = random_batch(16, seed=5)
         b1 = BatchVegas(specs)
-        assert b1.events_per_tile in autotune.EVENTS_PER_TILE
+        assert b1.events_per_tile in autotune.EVENTS_PER_TILE + autotune.PAIR_EVENTS_PER_TILE
         assert os.path.exists(path)
         autotune._memory.clear()
         b2 = BatchVegas(specs)
-        assert (b2.events_per_tile, b2.blocks) == (b1.events_per_tile, b1.blocks)
+        assert (b2.events_per_tile, b2.blocks, b2.pairs) == (b1.events_per_tile, b1.blocks, b1.pairs)
+        # mixed precision: the pair-histogram kernel competes too
+        b3 = BatchVegas(specs, dtype=torch.float32)
+        assert b3.events_per_tile in autotune.EVENTS_PER_TILE + autotune.PAIR_EVENTS_PER_TILE
     finally:
         os.environ.pop("MIVEGAS_TUNE_CACHE", None)
         if os.path.exists(path):

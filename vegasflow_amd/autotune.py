@@ -2,8 +2,9 @@
 
 For a batch signature (precision, families, the multiset of (n_dim,
 n_events) rounded to powers of two, and the device's compute-unit count) it
-times each candidate (events per tile, workgroups per compute unit) on
-scratch buffers that do not touch the batch's state, keeps the fastest, and
+times each candidate (events per tile, workgroups per compute unit, pair
+histograms or not) on scratch buffers that do not touch the batch's state,
+keeps the fastest, and
 remembers it in memory and, when ``MIVEGAS_TUNE_CACHE`` names a file, in that
 JSON file for later processes.
 """
@@ -15,6 +16,9 @@ import torch
 
 EVENTS_PER_TILE = (1024, 2048, 4096, 8192, 16384)
 BLOCKS_PER_CU = (2, 4, 8)
+# tiles of the pair-histogram kernel (1024-thread workgroups, as many as are
+# resident): larger, since a tile clears and sums its pair tables once
+PAIR_EVENTS_PER_TILE = (16384, 65536, 262144)
 
 _memory = {}
 _lock = threading.Lock()

@@ -2,8 +2,9 @@
 
 Each item is a VegasFlow integral (built-in integrand, n_dim <= 16, its own
 event count and Philox seed). One iteration round is one persistent launch
-of ``mv_batch_fused`` per integrand family present (workgroups stride over
-tiles of consecutive events of one item) plus one ``mv_batch_finalize``
+of ``mv_batch_fused`` (or, in mixed precision, ``mv_batch_fused_pairs``) per
+integrand family present (workgroups stride over tiles of consecutive events
+of one item) plus one ``mv_batch_finalize``
 (sums, sigma, running weighted average, history, grid refinement), with all
 state resident on the device. Item i draws exactly what
 ``VegasFlow(n_dim_i, n_events_i, seed=seed_i)`` would, so its results equal a
@@ -11,8 +12,8 @@ lone run up to summation order.
 
 Rounds can be captured once as a HIP graph (torch.cuda.CUDAGraph) and
 replayed: the per-item iteration counters live on the device, so nothing in a
-round depends on the host. The tile size and workgroup count come from the
-autotuner (vegasflow_amd.autotune) unless given.
+round depends on the host. The tile size, workgroup count and kernel come
+from the autotuner (vegasflow_amd.autotune) unless given.
 """
 import ctypes
 import hashlib
@@ -42,9 +43,14 @@ class BatchVegas:
         train: refine every item's grid after each iteration
         max_iter: iterations kept in the per-item history
         events_per_tile, blocks_per_cu: launch shape; autotuned when None
+        pair_histograms: mixed precision only: train through pair tables
+            (mv_batch_fused_pairs, 1024-thread workgroups, as many as are
+            resident; families whose max_dim it does not take fall back to
+            mv_batch_fused with blocks_per_cu); autotuned when None
     """
 
-    def __init__(self, specs, dtype=None, train=True, max_iter=64, events_per_tile=None, blocks_per_cu=None):
+    def __init__(self, specs, dtype=None, train=True, max_iter=64, events_per_tile=None, blocks_per_cu=None,
+                 pair_histograms=None):
         kernels.require_gpu()
         self.device = default_device()
         self.precision = precision_of(DTYPE if dtype is None else dtype)
@@ -98,13 +104,20 @@ class BatchVegas:
         self._graph = None
         self._warmed = False
 
-        if events_per_tile is None or blocks_per_cu is None:
-            ept, bpc = self._tune()
+        mixed = self.precision == "mixed"
+        if pair_histograms and not mixed:
+            raise ValueError("pair histograms are a mixed-precision kernel")
+        if events_per_tile is None or (blocks_per_cu is None and not pair_histograms):
+            ept, bpc, pairs = self._tune(pair_histograms)
             events_per_tile = events_per_tile or ept
             blocks_per_cu = blocks_per_cu or bpc
+            if pair_histograms is None:
+                pair_histograms = bool(pairs)
         self.events_per_tile = int(events_per_tile)
-        self.blocks = int(blocks_per_cu) * self._cus
+        self.pairs = bool(pair_histograms) and mixed
+        self.blocks = int(blocks_per_cu or autotune.BLOCKS_PER_CU[-1]) * self._cus
         self._launches = self._tile_launches(self.events_per_tile)
+        self._pair_blocks = self._pair_launch_blocks(self._launches) if self.pairs else {}
 
     # ------------------------------------------------------------ tiling
     def _tile_launches(self, events_per_tile):
@@ -129,38 +142,58 @@ class BatchVegas:
             launches.append((fam, dev_tiles, len(tiles), max_dim))
         return launches
 
+    @staticmethod
+    def _pair_launch_blocks(launches):
+        """Resident pair-kernel workgroups per family launch (0: not taken)."""
+        return {fam: kernels.batch_pairs_blocks(fam, maxd) for fam, _, _, maxd in launches}
+
     def _signature(self):
         h = hashlib.sha1()
         pairs = sorted((fn.family, d, _bucket(n)) for fn, d, n, _ in self.specs)
         h.update(repr(pairs).encode())
         return f"batch|{self.precision}|{int(self.train)}|cus={self._cus}|{h.hexdigest()[:16]}"
 
-    def _tune(self):
+    def _launch_round(self, launches, blocks, pair_blocks, rows):
+        for fam, tiles, nt, maxd in launches:
+            pb = pair_blocks.get(fam, 0)
+            if pb > 0:
+                kernels.batch_fused_pairs(self.items, self.n_items, tiles, nt, fam, maxd, self.draws,
+                                          self.grids, rows, pb)
+            else:
+                kernels.batch_fused(self.items, self.n_items, tiles, nt, self.precision, fam, maxd,
+                                    self.draws, self.grids, rows, blocks)
+
+    def _tune(self, pair_histograms=None):
+        """Fastest (events per tile, workgroups per CU, pairs) for this batch.
+        pair_histograms None: both kernels compete (mixed only); True / False:
+        only the pair / per-wave candidates."""
         scratch = torch.zeros_like(self.rows)
 
         def measure(cand):
-            ept, bpc = cand
+            ept, bpc, pairs = cand
             launches = self._tile_launches(ept)
+            pair_blocks = self._pair_launch_blocks(launches) if pairs else {}
+            bpc = bpc or autotune.BLOCKS_PER_CU[-1]
 
             def launch():
-                for fam, tiles, nt, maxd in launches:
-                    kernels.batch_fused(self.items, self.n_items, tiles, nt, self.precision, fam, maxd,
-                                        self.draws, self.grids, scratch, bpc * self._cus)
+                self._launch_round(launches, bpc * self._cus, pair_blocks, scratch)
 
             ms = autotune.time_ms(launch)
             scratch.zero_()
             return ms
 
-        cands = [(e, b) for e in autotune.EVENTS_PER_TILE for b in autotune.BLOCKS_PER_CU]
-        choice = autotune.tune(self._signature(), cands, measure)
-        logger.info("batch launch shape: %d events per tile, %d workgroups per CU", *choice)
+        cands = []
+        if not pair_histograms:
+            cands += [(e, b, 0) for e in autotune.EVENTS_PER_TILE for b in autotune.BLOCKS_PER_CU]
+        if self.precision == "mixed" and pair_histograms is not False:
+            cands += [(e, 0, 1) for e in autotune.PAIR_EVENTS_PER_TILE]
+        choice = autotune.tune(self._signature() + f"|pairs={pair_histograms}", cands, measure)
+        logger.info("batch launch shape: %d events per tile, %d workgroups per CU, pairs %d", *choice)
         return choice
 
     # ------------------------------------------------------------ rounds
     def _round(self):
-        for fam, tiles, nt, maxd in self._launches:
-            kernels.batch_fused(self.items, self.n_items, tiles, nt, self.precision, fam, maxd, self.draws,
-                                self.grids, self.rows, self.blocks)
+        self._launch_round(self._launches, self.blocks, self._pair_blocks, self.rows)
         kernels.batch_finalize(self.items, self.jobs, self.n_jobs, self.grids, self.rows, self.draws,
                                self.stats, self.history_buf, self.max_iter)
 
@@ -172,8 +205,12 @@ class BatchVegas:
         rows, draws = self.rows.clone(), self.draws.clone()
         grids, stats, hist = self.grids.clone(), self.stats.clone(), self.history_buf.clone()
         for fam, tiles, nt, maxd in self._launches:
-            kernels.batch_fused(self.items, self.n_items, tiles, nt, self.precision, fam, maxd, draws, grids,
-                                rows, self.blocks)
+            if self._pair_blocks.get(fam, 0) > 0:
+                kernels.batch_fused_pairs(self.items, self.n_items, tiles, nt, fam, maxd, draws, grids, rows,
+                                          self._pair_blocks[fam])
+            else:
+                kernels.batch_fused(self.items, self.n_items, tiles, nt, self.precision, fam, maxd, draws,
+                                    grids, rows, self.blocks)
         kernels.batch_finalize(self.items, self.jobs, self.n_jobs, grids, rows, draws, stats, hist,
                                self.max_iter)
         torch.cuda.synchronize()

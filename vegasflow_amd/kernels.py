@@ -291,6 +291,19 @@ def batch_fused(items_dev, n_items, tiles_dev, n_tiles, precision, family, max_d
               int(blocks), _stream())
 
 
+def batch_fused_pairs(items_dev, n_items, tiles_dev, n_tiles, family, max_dim, draws, grids, rows, blocks):
+    """mv_batch_fused_pairs (mixed precision, pair histograms, 1024-thread
+    workgroups); arguments as batch_fused."""
+    _abi.call("mv_batch_fused_pairs", _ptr(items_dev), int(n_items), _ptr(tiles_dev), int(n_tiles),
+              int(family), int(max_dim), _ptr(draws), _ptr(grids), _ptr(rows), int(blocks), _stream())
+
+
+def batch_pairs_blocks(family, max_dim):
+    """Resident 1024-thread workgroups of mv_batch_fused_pairs on the current
+    device for this family and max_dim (0: the pair kernel does not apply)."""
+    return int(_abi.load().mv_batch_pairs_blocks(int(family), int(max_dim)))
+
+
 def batch_finalize(items_dev, jobs, n_jobs, grids, rows, draws, stats, history=None, max_iter=0):
     _abi.call("mv_batch_finalize", _ptr(items_dev), _ptr(jobs), int(n_jobs), _ptr(grids), _ptr(rows),
               _ptr(draws), _ptr(stats), _ptr(history), int(max_iter), _stream())
