@@ -212,7 +212,7 @@ def bench_batched(torch, dtype, n_iter=8):
             "median_chi2_dof": float(np.nanmedian(b.chi2_dof()))}
 
 
-def bench_serving(seconds, rate=200.0):
+def bench_serving(seconds, dtype, rate=200.0):
     """Poisson stream at ``rate`` req/s for ``seconds``: two-Gaussian mixture
     (d=4) or Breit-Wigner x power (d=6); target 1e-3 (1e6 events/iteration)
     or 1e-4 (2e7 events/iteration); at most 20 iterations."""
@@ -220,7 +220,7 @@ def bench_serving(seconds, rate=200.0):
     from vegasflow_amd.serve import Server, latency_report, poisson_requests
 
     reqs = poisson_requests(rate=rate, duration=seconds, seed=5)
-    server = Server(memory_budget=256 << 20)
+    server = Server(memory_budget=256 << 20, dtype=dtype)
     t0 = time.perf_counter()
     done = server.run_stream(reqs)
     dt = time.perf_counter() - t0
@@ -229,7 +229,8 @@ def bench_serving(seconds, rate=200.0):
     rep.update({"rate_req_per_s": rate, "duration_s": seconds, "wall_s": dt,
                 "events_per_s": server.events_done / dt, "ticks": server.ticks,
                 "tick_ms_p50": float(np.percentile(ticks, 50)), "tick_ms_p99": float(np.percentile(ticks, 99)),
-                "tick_ms_max": float(ticks.max()), "slots": server.capacity})
+                "tick_ms_max": float(ticks.max()), "slots": server.capacity,
+                "pair_histograms": server.pairs})
     return rep
 
 
@@ -359,7 +360,7 @@ def main():
         elif args.only == "batch":
             out = {p: bench_batched(torch, dt_) for p, dt_ in precisions}
         else:
-            out = bench_serving(args.serve_seconds)
+            out = {p: bench_serving(args.serve_seconds, dt_) for p, dt_ in precisions}
         print(json.dumps({"section": args.only, "result": out}))
         return 0
     fn = vf.Gaussian(A, 0.5)
@@ -449,7 +450,8 @@ def main():
                                     (("mixed", torch.float32), ("f64", torch.float64))}
         line["batched_256"] = {p: bench_batched(torch, dt_) for p, dt_ in
                                (("mixed", torch.float32), ("f64", torch.float64))}
-        line["serving"] = bench_serving(args.serve_seconds)
+        line["serving"] = {p: bench_serving(args.serve_seconds, dt_) for p, dt_ in
+                           (("mixed", torch.float32), ("f64", torch.float64))}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))

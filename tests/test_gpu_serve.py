@@ -27,13 +27,16 @@ def mixed_requests(n, seed=0):
     return out
 
 
-def test_results_same_alone_and_in_the_mix():
-    server = Server(memory_budget=64 << 20)
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_results_same_alone_and_in_the_mix(dtype):
+    """FP64, and mixed precision through the pair-histogram batch kernel."""
+    server = Server(memory_budget=64 << 20, dtype=dtype)
+    assert server.pairs == (dtype == torch.float32)
     done = server.run_stream(mixed_requests(18))
     assert len(done) == 18
     for req in done:
         assert req.iterations >= server.min_iter
-        res, err = run_alone(req)
+        res, err = run_alone(req, dtype=dtype)
         assert abs(req.result - res) <= 1e-9 * abs(res), req.rid
         assert abs(req.error - err) <= 1e-6 * abs(err), req.rid
 

@@ -65,10 +65,16 @@ class Server:
     """Request loop over a slab of ``capacity`` slots (from memory_budget)."""
 
     def __init__(self, memory_budget=64 << 20, dtype=None, min_iter=3, events_per_tile=16384,
-                 blocks_per_cu=4):
+                 blocks_per_cu=4, pair_histograms=None, pair_events_per_tile=65536):
         kernels.require_gpu()
         self.device = default_device()
         self.precision = precision_of(DTYPE if dtype is None else dtype)
+        # mixed precision trains through the pair-histogram batch kernel
+        # (mv_batch_fused_pairs) wherever it takes the family's max n_dim
+        self.pairs = self.precision == "mixed" if pair_histograms is None else bool(pair_histograms)
+        if self.pairs and self.precision != "mixed":
+            raise ValueError("pair histograms are a mixed-precision kernel")
+        self.pair_events_per_tile = int(pair_events_per_tile)
         self.capacity = int(memory_budget // slot_bytes())
         if self.capacity < 1:
             raise ValueError("memory budget below one slot")
@@ -150,21 +156,26 @@ class Server:
             by_family[req.integrand.family].append((s, req))
             jobs.append((s, -1))
             jobs.extend((s, j) for j in range(req.n_dim))
-        # tiles of at most events_per_tile events, smaller (powers of two down
-        # to 1024) when the live requests are too few to give every
-        # workgroup two tiles, so that a light load still spreads over the GPU
+        # tiles of at most events_per_tile (pair kernel: pair_events_per_tile)
+        # events, smaller (powers of two) when the live requests are too few to
+        # give every workgroup two tiles, so that a light load still spreads
+        # over the GPU
         total = sum(self.live[s].n_events for s in slots)
-        ept = self.events_per_tile
-        while ept > 1024 and total < 2 * self.blocks * ept:
-            ept //= 2
         if len(self._tile_cache) > 4 * self.capacity:
             self._tile_cache.clear()
         fams = []
         for fam, members in sorted(by_family.items()):
+            max_dim = max(r.n_dim for _, r in members)
+            pb = kernels.batch_pairs_blocks(fam, max_dim) if self.pairs else 0
+            if pb > 0:
+                ept, blocks, floor = self.pair_events_per_tile, pb, 4096
+            else:
+                ept, blocks, floor = self.events_per_tile, self.blocks, 1024
+            while ept > floor and total < 2 * blocks * ept:
+                ept //= 2
             tiles = np.concatenate([self._tiles_of(s, r, ept) for s, r in members])
             tiles_dev = torch.from_numpy(tiles.view(np.uint8)).to(self.device)
-            fams.append((fam, tiles_dev, len(tiles), max(r.n_dim for _, r in members),
-                         sum(r.n_events for _, r in members)))
+            fams.append((fam, tiles_dev, len(tiles), max_dim, sum(r.n_events for _, r in members), pb))
         jobs_dev = torch.tensor(np.array(jobs, dtype=np.int32).reshape(-1), device=self.device)
         slots_dev = torch.tensor(slots, dtype=torch.int64, device=self.device)
         self._launch = (fams, jobs_dev, len(jobs), slots, slots_dev)
@@ -177,9 +188,13 @@ class Server:
         if self._launch is None:
             self._build_launch()
         fams, jobs_dev, n_jobs, slots, slots_dev = self._launch
-        for fam, tiles_dev, n_tiles, max_dim, events in fams:
-            kernels.batch_fused(self.items, self.capacity, tiles_dev, n_tiles, self.precision, fam, max_dim,
-                                self.draws, self.grids, self.rows, self.blocks)
+        for fam, tiles_dev, n_tiles, max_dim, events, pb in fams:
+            if pb > 0:
+                kernels.batch_fused_pairs(self.items, self.capacity, tiles_dev, n_tiles, fam, max_dim,
+                                          self.draws, self.grids, self.rows, pb)
+            else:
+                kernels.batch_fused(self.items, self.capacity, tiles_dev, n_tiles, self.precision, fam,
+                                    max_dim, self.draws, self.grids, self.rows, self.blocks)
             self.events_done += events
         kernels.batch_finalize(self.items, jobs_dev, n_jobs, self.grids, self.rows, self.draws, self.stats)
         # the live slots' counters and running sums only (one small copy; synchronises)
