@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MV_ABI_VERSION 1
+#define MV_ABI_VERSION 2
 #define MV_BINS 50             /* BINS_MAX, configflow.py:13 */
 #define MV_FUSED_MAX_DIM 16    /* largest n_dim the fused kernels take */
 #define MV_MAX_COMP 4          /* components of the built-in mixture family */
@@ -80,7 +80,10 @@ typedef struct {
  * is r = ((hi:lo) >> 12) * 2^-52 mapped to [1e-8, 1 - 1e-8]
  * (monte_carlo.py:264-266); in MV_MIXED it is the 32-bit word hi alone.
  * ``inject`` (nullable) replaces the generator: FP64 uniforms (n_events,
- * n_dim) in MV_F64, uint32 hi words (n_events, n_dim) in MV_MIXED. */
+ * n_dim) in MV_F64, uint32 hi words (n_events, n_dim) in MV_MIXED.
+ * ``draw_dev`` (nullable) points to a device uint32 that holds the draw id
+ * instead of ``draw_id``: with the counter of mv_finalize / mv_vplus_finalize
+ * a captured iteration can be replayed with a new draw each time. */
 typedef struct {
   int32_t n_dim;
   int32_t precision;     /* mv_precision */
@@ -96,6 +99,7 @@ typedef struct {
   const double* xmin;    /* nullable: integration limits (n_dim) */
   const double* xdelta;  /* nullable: xmax - xmin (n_dim) */
   const void* inject;    /* nullable, see above */
+  const uint32_t* draw_dev;  /* nullable, see above */
 } mv_step;
 
 /* Caller-owned partial-sum workspace: MV_ROWS rows of ``stride`` doubles,
@@ -127,6 +131,15 @@ typedef struct {
   int32_t raw;           /* write sum tmp^2 instead of sigma */
   int32_t reserved;
   double n_events;       /* N of the sigma formula */
+  /* Iteration counter (nullable): draw_dev[0] is the draw id of the
+   * iteration being finalized, draw_dev[1] a base. After the results,
+   * {res, sigma, events} of output 0 go to history[3 k .. 3 k + 2] with
+   * k = draw_dev[0] - draw_dev[1] when 0 <= k < history_len (history
+   * nullable; events = n_events), and draw_dev[0] is incremented. */
+  uint32_t* draw_dev;
+  double* history;
+  int32_t history_len;
+  int32_t reserved2;
 } mv_finalize;
 
 /* ---- library ---------------------------------------------------------- */
@@ -226,12 +239,15 @@ int64_t mv_vplus_out_len(int32_t n_cubes);
  * n_ev = int(max(min_neval, damped * init_calls / 2 / sum damped))); ress
  * and ress2 are zeroed; with train the histogram rows are reduced into
  * hist_out (nullable) and, with refine, the grid is refined. arr_var_out
- * (nullable, n_cubes) receives ress2 * n_ev - ress^2. */
+ * (nullable, n_cubes) receives ress2 * n_ev - ress^2. draw_dev, history,
+ * history_len: the iteration counter of mv_finalize (nullable); the events
+ * recorded are those of the iteration just finalized, offsets[n_cubes]. */
 int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t* n_ev, int32_t* offsets,
                       double* ress, double* ress2, int32_t adaptive, int32_t min_neval,
                       double init_calls, double beta, double* rows, int64_t row_stride,
                       int32_t n_rows, int32_t train, int32_t refine, double* grid, double* out,
-                      double* arr_var_out, double* hist_out, void* stream);
+                      double* arr_var_out, double* hist_out, uint32_t* draw_dev,
+                      double* history, int32_t history_len, void* stream);
 
 /* ---- batched independent integrals ------------------------------------- */
 /* No counterpart in the reference (each VegasFlow instance there runs its

@@ -10,7 +10,7 @@ import threading
 
 from . import _build
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 BINS = 50
 FUSED_MAX_DIM = 16
 MAX_COMP = 4
@@ -68,6 +68,7 @@ class Step(ctypes.Structure):
         ("xmin", _vp),
         ("xdelta", _vp),
         ("inject", _vp),
+        ("draw_dev", _vp),
     ]
 
 
@@ -96,6 +97,10 @@ class Finalize(ctypes.Structure):
         ("raw", _i32),
         ("reserved", _i32),
         ("n_events", _f64),
+        ("draw_dev", _vp),
+        ("history", _vp),
+        ("history_len", _i32),
+        ("reserved2", _i32),
     ]
 
 
@@ -147,7 +152,7 @@ SIGNATURES = {
     "mv_vplus_out_len": (_i64, [_i32]),
     "mv_vplus_finalize": (ctypes.c_int, [_i32, _P(Strata), _vp, _vp, _vp, _vp, _i32, _i32, _f64,
                                          _f64, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
-                                         _vp]),
+                                         _vp, _vp, _i32, _vp]),
     "mv_batch_fused": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "mv_batch_fused_pairs": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "mv_batch_pairs_blocks": (_i32, [_i32, _i32]),

@@ -220,10 +220,7 @@ class BatchVegas:
         """Capture one round as a HIP graph (done once)."""
         if self._graph is None:
             self._warm_up()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                self._round()
-            self._graph = g
+            self._graph, _ = kernels.capture_graph(self._round)
         return self._graph
 
     def step(self, graph=False):

@@ -60,9 +60,32 @@ __device__ inline void refine_wave(const double* h /* LDS, 50 */, double* __rest
   if (lane <= kBins) div[lane] = mine;
 }
 
+// Iteration counter advanced by the finalize kernels (mv_finalize): one
+// thread stores {res, sigma, events} in history slot draw[0] - draw[1] when
+// that is below len, then moves draw[0] to the next draw id.
+struct CounterK {
+  uint32_t* draw;
+  double* history;
+  int len;
+};
+
+__device__ __forceinline__ void advance_counter(const CounterK& c, double res, double sig,
+                                                double events) {
+  if (!c.draw) return;
+  const uint32_t k = c.draw[0] - c.draw[1];
+  if (c.history && k < (uint32_t)c.len) {
+    double* h = c.history + 3 * (int64_t)k;
+    h[0] = res;
+    h[1] = sig;
+    h[2] = events;
+  }
+  c.draw[0] = c.draw[0] + 1u;
+}
+
 struct FzK {
   int n_dim, n_out, train, refine, raw;
   double n_events;
+  CounterK counter;  // zero: no counter
 };
 
 // Sum column c of the (at most MV_ROWS) group rows in order and zero it for
@@ -96,12 +119,13 @@ __device__ inline void finalize_sums_block(const FzK& z, double* __restrict__ P,
     const double res = red[t];
     const double res2 = red[z.n_out + t];
     out[t] = res;
-    if (z.raw) {
-      out[z.n_out + t] = res2;
-    } else {
+    double second = res2;
+    if (!z.raw) {
       const double err = (N * res2 - res * res) / (N - 1.0);  // vflow.py:437
-      out[z.n_out + t] = sqrt(fmax(err, 0.0));                // vflow.py:438
+      second = sqrt(fmax(err, 0.0));                          // vflow.py:438
     }
+    out[z.n_out + t] = second;
+    if (t == 0) advance_counter(z.counter, res, second, N);
   }
 }
 

@@ -31,6 +31,7 @@ struct StepK {
   uint32_t ev_lo, ev_hi;
   Key key;
   uint32_t draw_id;
+  const uint32_t* draw_ptr;  // nullable: device word holding the draw id
   double xjac;
   double xdeltajac;
   const double* xmin;
@@ -49,12 +50,20 @@ inline StepK make_stepk(const mv_step& s) {
   k.key.k0 = (uint32_t)(s.seed & 0xffffffffu);
   k.key.k1 = (uint32_t)(s.seed >> 32);
   k.draw_id = s.draw_id;
+  k.draw_ptr = s.draw_dev;
   k.xjac = s.xjac;
   k.xdeltajac = s.xdeltajac;
   k.xmin = s.xmin;
   k.xdelta = s.xdelta;
   k.inject = s.inject;
   return k;
+}
+
+// The draw id of a launch: the device word draw_ptr points to (a counter
+// that finalize kernels advance, so that captured graphs can be replayed),
+// else the host value. Read once per kernel, before the event loop.
+__device__ __forceinline__ uint32_t draw_of(const StepK& s) {
+  return s.draw_ptr ? *s.draw_ptr : s.draw_id;
 }
 
 // Run ``launch(StepK, first)`` over pieces of the step's events that do not

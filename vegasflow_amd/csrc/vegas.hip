@@ -76,6 +76,7 @@ template <int FAM, bool TRAIN>
 __global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand fn,
                                                           const double* __restrict__ grid,
                                                           RowsK R) {
+  const uint32_t draw = draw_of(s);
   extern __shared__ double smem[];
   const int d = s.n_dim;
   const int nb = d * kBins;
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand 
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const double tmp = event_f64<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id, s.key,
+    const double tmp = event_f64<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key,
                                              s.groups, d, tab, tw, wh, inj ? inj + i * d : nullptr,
                                              wscale);
     acc += tmp;
@@ -109,6 +110,7 @@ template <int FAM, bool TRAIN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void vegas_fused_mixed(StepK s, mv_integrand fn,
                                                             const double* __restrict__ grid,
                                                             RowsK R) {
+  const uint32_t draw = draw_of(s);
   extern __shared__ double smem[];
   const int d = s.n_dim;
   const int nb = d * kBins;
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const double tmp = event_mixed<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id, s.key,
+    const double tmp = event_mixed<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key,
                                                s.groups, d, tab, cprod, wh, inj ? inj + i * d : nullptr,
                                                wscale);
     acc += tmp;
@@ -145,6 +147,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 template <int FAM, int BS>
 __global__ __launch_bounds__(BS) void vegas_fused_pairs(StepK s, mv_integrand fn,
                                                         const double* __restrict__ grid, RowsK R) {
+  const uint32_t draw = draw_of(s);
   extern __shared__ double smem[];
   const int d = s.n_dim;
   const int nb = d * kBins;
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(BS) void vegas_fused_pairs(StepK s, mv_integrand fn
     }
   } else {
     for (int64_t i = i0; i < s.n_events; i += step) {
-      const double tmp = event_mixed_pairs<FAM, false>(F, s.ev_lo + (uint32_t)i, s.ev_hi, s.draw_id,
+      const double tmp = event_mixed_pairs<FAM, false>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw,
                                                        s.key, d, tab, cprod, H, nullptr, wscale);
       acc += tmp;
       acc2 += tmp * tmp;
@@ -184,6 +187,7 @@ __global__ __launch_bounds__(BS) void vegas_fused_pairs(StepK s, mv_integrand fn
 // ---------------------------------------------------------------------------
 template <int FAM, bool MIXED>
 __global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, RowsK R) {
+  const uint32_t draw = draw_of(s);
   using T = typename std::conditional<MIXED, float, double>::type;
   Integrand<T, FAM> F(fn);
   const bool limits = s.xmin != nullptr;
@@ -196,8 +200,8 @@ __global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, 
     for (int g = 0; g < s.groups; ++g) {
       uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
       if (!s.inject) {
-        h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-        if (!MIXED) l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+        h = philox(c0, c1, draw, (uint32_t)g, s.key);
+        if (!MIXED) l = philox(c0, c1, draw, (uint32_t)(s.groups + g), s.key);
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -266,6 +270,7 @@ template <bool MIXED, bool STAGED>
 __global__ __launch_bounds__(kBlock) void vegas_sample(StepK s, const double* __restrict__ grid,
                                                        void* __restrict__ xo_, void* __restrict__ wo_,
                                                        uint8_t* __restrict__ io) {
+  const uint32_t draw = draw_of(s);
   using T = typename std::conditional<MIXED, float, double>::type;
   extern __shared__ double stage_mem[];
   T* xo = static_cast<T*>(xo_);
@@ -284,8 +289,8 @@ __global__ __launch_bounds__(kBlock) void vegas_sample(StepK s, const double* __
       for (int g = 0; g < s.groups; ++g) {
         uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
         if (!s.inject) {
-          h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-          if (!MIXED) l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+          h = philox(c0, c1, draw, (uint32_t)g, s.key);
+          if (!MIXED) l = philox(c0, c1, draw, (uint32_t)(s.groups + g), s.key);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -345,6 +350,7 @@ __global__ __launch_bounds__(kBlock) void vegas_sample(StepK s, const double* __
 // monte_carlo.py:264-274 (PlainFlow draws): x = u, or xmin + u * xdelta.
 template <bool MIXED, bool STAGED>
 __global__ __launch_bounds__(kBlock) void plain_sample(StepK s, void* __restrict__ xo_) {
+  const uint32_t draw = draw_of(s);
   using T = typename std::conditional<MIXED, float, double>::type;
   extern __shared__ double stage_mem[];
   T* xo = static_cast<T*>(xo_);
@@ -360,8 +366,8 @@ __global__ __launch_bounds__(kBlock) void plain_sample(StepK s, void* __restrict
       for (int g = 0; g < s.groups; ++g) {
         uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
         if (!s.inject) {
-          h = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-          if (!MIXED) l = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+          h = philox(c0, c1, draw, (uint32_t)g, s.key);
+          if (!MIXED) l = philox(c0, c1, draw, (uint32_t)(s.groups + g), s.key);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -782,7 +788,9 @@ extern "C" int mv_vegas_finalize(const mv_finalize* fz, double* rows, int64_t ro
   if (fz->refine && (!fz->train || !grid))
     return fail(MV_ERR_ARG, "mv_vegas_finalize: refine needs train rows and a grid");
   if (hist_out && !fz->train) return fail(MV_ERR_ARG, "mv_vegas_finalize: no histogram without train");
-  FzK z{fz->n_dim, fz->n_out, fz->train, fz->refine, fz->raw, fz->n_events};
+  if (fz->history && fz->history_len < 0) return fail(MV_ERR_ARG, "mv_vegas_finalize: history_len < 0");
+  FzK z{fz->n_dim, fz->n_out, fz->train, fz->refine, fz->raw, fz->n_events,
+        CounterK{fz->draw_dev, fz->history, fz->history_len}};
   const int blocks = 1 + (fz->train ? fz->n_dim : 0);  // histogram columns are always consumed
   hipLaunchKernelGGL(finalize_kernel, dim3(blocks), dim3(kBlock), 0, as_stream(stream), z, rows,
                      row_stride, n_rows, grid, out, hist_out);

@@ -101,6 +101,7 @@ __global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_in
                                                       const double* __restrict__ grid,
                                                       double* __restrict__ ress,
                                                       double* __restrict__ ress2, RowsK R) {
+  const uint32_t draw = draw_of(s);
   using T = typename std::conditional<MIXED, float, double>::type;
   using T2 = typename std::conditional<MIXED, float2, double2>::type;
   extern __shared__ double smem[];
@@ -157,8 +158,8 @@ __global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_in
         if (g < s.groups) {
           uint4 hw4 = make_uint4(0, 0, 0, 0), lw4 = make_uint4(0, 0, 0, 0);
           if (!s.inject) {
-            hw4 = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-            if (!MIXED) lw4 = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+            hw4 = philox(c0, c1, draw, (uint32_t)g, s.key);
+            if (!MIXED) lw4 = philox(c0, c1, draw, (uint32_t)(s.groups + g), s.key);
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
@@ -207,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void vplus_sample(StepK s, StrataK st,
                                                        void* __restrict__ xo_, void* __restrict__ wo_,
                                                        uint8_t* __restrict__ io,
                                                        int32_t* __restrict__ so) {
+  const uint32_t draw = draw_of(s);
   const int d = s.n_dim;
   const bool limits = s.xmin != nullptr;
   const double fstrat = (double)st.n_strat;
@@ -235,8 +237,8 @@ __global__ __launch_bounds__(kBlock) void vplus_sample(StepK s, StrataK st,
       }
       uint4 hw4 = make_uint4(0, 0, 0, 0), lw4 = make_uint4(0, 0, 0, 0);
       if (!s.inject) {
-        hw4 = philox(c0, c1, s.draw_id, (uint32_t)g, s.key);
-        if (!MIXED) lw4 = philox(c0, c1, s.draw_id, (uint32_t)(s.groups + g), s.key);
+        hw4 = philox(c0, c1, draw, (uint32_t)g, s.key);
+        if (!MIXED) lw4 = philox(c0, c1, draw, (uint32_t)(s.groups + g), s.key);
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -421,9 +423,10 @@ __device__ __forceinline__ void cube_totals(const double* __restrict__ out, int 
 // init_calls / 2 / sum damped)) when adaptive (and the sum is positive), the
 // events per workgroup, and out[0..1] = {res, sigma} (vflowplus.py:231-233).
 __global__ __launch_bounds__(kBlock) void vplus_realloc(int n_cubes, int32_t* __restrict__ n_ev,
+                                                        const int32_t* __restrict__ offsets,
                                                         const double* __restrict__ ress2, int adaptive,
                                                         int min_neval, double init_calls,
-                                                        double* __restrict__ out) {
+                                                        double* __restrict__ out, CounterK counter) {
   __shared__ double tot[3];
   __shared__ double part[kWaves];
   const int nb = (n_cubes + kBlock - 1) / kBlock;
@@ -445,6 +448,8 @@ __global__ __launch_bounds__(kBlock) void vplus_realloc(int n_cubes, int32_t* __
     if (blockIdx.x == 0) {
       out[0] = tot[0];
       out[1] = sqrt(tot[1]);  // vflowplus.py:233
+      // this iteration's events: offsets are rewritten only by vplus_offsets
+      advance_counter(counter, out[0], out[1], (double)offsets[n_cubes]);
     }
   }
 }
@@ -603,9 +608,12 @@ extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t
                                  double* ress, double* ress2, int32_t adaptive, int32_t min_neval,
                                  double init_calls, double beta, double* rows, int64_t row_stride,
                                  int32_t n_rows, int32_t train, int32_t refine, double* grid,
-                                 double* out, double* arr_var_out, double* hist_out, void* stream) {
+                                 double* out, double* arr_var_out, double* hist_out,
+                                 uint32_t* draw_dev, double* history, int32_t history_len,
+                                 void* stream) {
   int rc = validate_strata(strata, n_dim, "mv_vplus_finalize");
   if (rc) return rc;
+  if (history && history_len < 0) return fail(MV_ERR_ARG, "mv_vplus_finalize: history_len < 0");
   if (!n_ev || !offsets || !ress || !ress2 || !out)
     return fail(MV_ERR_ARG, "mv_vplus_finalize: missing buffer");
   if (adaptive && min_neval < 1) return fail(MV_ERR_ARG, "mv_vplus_finalize: min_neval < 1");
@@ -623,8 +631,8 @@ extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t
   hipLaunchKernelGGL(vplus_cube_stats, dim3(nb), dim3(kBlock), 0, s, n_cubes, n_ev, ress, ress2, adaptive,
                      beta, out, arr_var_out);
   if ((rc = check_launch("mv_vplus_finalize"))) return rc;
-  hipLaunchKernelGGL(vplus_realloc, dim3(nb), dim3(kBlock), 0, s, n_cubes, n_ev, ress2, adaptive, min_neval,
-                     init_calls, out);
+  hipLaunchKernelGGL(vplus_realloc, dim3(nb), dim3(kBlock), 0, s, n_cubes, n_ev, offsets, ress2, adaptive,
+                     min_neval, init_calls, out, CounterK{draw_dev, history, history_len});
   if ((rc = check_launch("mv_vplus_finalize"))) return rc;
   const FzK z{n_dim, 1, 1, refine, 0, 0.0};
   hipLaunchKernelGGL(vplus_offsets, dim3(nb + (train ? n_dim : 0)), dim3(kBlock), 0, s, n_cubes, n_ev,

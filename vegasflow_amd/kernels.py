@@ -7,6 +7,7 @@ integrator classes are built on these functions, and the parity tests call
 them directly (e.g. with injected draws).
 """
 import ctypes
+import gc
 
 import torch
 
@@ -92,8 +93,10 @@ def producer_rows(n_events):
 
 
 def make_step(n_dim, n_events, *, precision="f64", train=True, event_offset=0,
-              seed=0, draw_id=0, xjac=1.0, xmin=None, xdelta=None, xdeltajac=1.0, inject=None):
+              seed=0, draw_id=0, xjac=1.0, xmin=None, xdelta=None, xdeltajac=1.0, inject=None,
+              draw_dev=None):
     _need_device(xmin, "xmin", torch.float64)
+    _need_device(draw_dev, "draw_dev", torch.int32)
     _need_device(xdelta, "xdelta", torch.float64)
     if inject is not None:
         want = torch.float64 if precision_code(precision) == _abi.F64 else torch.int32
@@ -115,11 +118,51 @@ def make_step(n_dim, n_events, *, precision="f64", train=True, event_offset=0,
     s.xmin = None if xmin is None else xmin.data_ptr()
     s.xdelta = None if xdelta is None else xdelta.data_ptr()
     s.inject = None if inject is None else inject.data_ptr()
+    s.draw_dev = None if draw_dev is None else draw_dev.data_ptr()
     # the struct holds raw addresses: keep the tensors alive with it, so that
     # temporaries passed here are not freed (and their memory handed to a
     # later allocation) before the launch that uses the struct
-    s._tensors = (xmin, xdelta, inject)
+    s._tensors = (xmin, xdelta, inject, draw_dev)
     return s
+
+
+def capture_graph(body):
+    """Capture ``body()`` (launches on the current stream) as a HIP graph;
+    returns (graph, what body returned). Python's cyclic garbage collector is
+    off while the capture runs: a collection there can destroy an unreachable
+    graph or release device memory, operations a stream capture does not
+    allow (torch collects once, before the capture begins)."""
+    graph = torch.cuda.CUDAGraph()
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        with torch.cuda.graph(graph):
+            out = body()
+    finally:
+        if enabled:
+            gc.enable()
+    return graph, out
+
+
+class IterationCounter:
+    """Device iteration counter for replayed iterations: ``draw`` (int32 x 2)
+    holds {draw id of the next iteration, draw id of history slot 0};
+    steps read the draw id from it and the finalize kernels store {res,
+    sigma, events} of each iteration in ``history`` (float64, 3 * capacity)
+    and advance it."""
+
+    def __init__(self, device, capacity):
+        self.draw = torch.zeros(2, dtype=torch.int32, device=device)
+        self.capacity = int(capacity)
+        self.history = torch.zeros(3 * self.capacity, dtype=torch.float64, device=device)
+
+    def start(self, draw_id):
+        """Next iteration draws ``draw_id`` and lands in history slot 0."""
+        self.draw.copy_(torch.tensor([draw_id, draw_id], dtype=torch.int32))
+
+    def results(self, n):
+        """(n, 3) device view of the first n iterations' {res, sigma, events}."""
+        return self.history[: 3 * n].view(n, 3)
 
 
 # ---------------------------------------------------------------------------
@@ -177,9 +220,10 @@ def vegas_digest(step, f, w, idx, ws, main_dim=0):
 
 
 def vegas_finalize(n_dim, n_out, train, refine, n_events, ws, n_rows, grid=None, hist_out=None,
-                   raw=False):
+                   raw=False, counter=None):
     """Reduce group rows; returns a (2 * n_out,) float64 tensor [res, sigma]
-    (or [res, sum tmp^2] with raw)."""
+    (or [res, sum tmp^2] with raw). ``counter``: an IterationCounter the
+    kernel records the result in and advances."""
     out = torch.empty(2 * n_out, dtype=torch.float64, device=ws.device)
     _need_device(grid, "grid", torch.float64)
     _need_device(hist_out, "hist_out", torch.float64)
@@ -190,6 +234,10 @@ def vegas_finalize(n_dim, n_out, train, refine, n_events, ws, n_rows, grid=None,
     fz.refine = int(bool(refine))
     fz.raw = int(bool(raw))
     fz.n_events = float(n_events)
+    if counter is not None:
+        fz.draw_dev = counter.draw.data_ptr()
+        fz.history = counter.history.data_ptr()
+        fz.history_len = counter.capacity
     _abi.call("mv_vegas_finalize", ctypes.byref(fz), _ptr(ws.rows), ws.stride, int(n_rows),
               _ptr(grid), _ptr(out), _ptr(hist_out), _stream())
     return out
@@ -268,14 +316,18 @@ def vplus_digest(step, strata, f, w, idx, seg, ress, ress2, ws):
 
 
 def vplus_finalize(n_dim, strata, n_ev, offsets, ress, ress2, adaptive, min_neval, init_calls, beta,
-                   ws, n_rows, train, refine, grid, arr_var_out=None, hist_out=None):
-    """Returns a (3,) float64 tensor: res, sigma, events of the next iteration."""
+                   ws, n_rows, train, refine, grid, arr_var_out=None, hist_out=None, counter=None):
+    """Returns a (3,) float64 tensor: res, sigma, events of the next iteration.
+    ``counter``: an IterationCounter, as in vegas_finalize."""
     out = torch.empty(int(_abi.load().mv_vplus_out_len(int(strata.n_cubes))), dtype=torch.float64,
                       device=ws.device)
     _abi.call("mv_vplus_finalize", int(n_dim), ctypes.byref(strata), _ptr(n_ev), _ptr(offsets), _ptr(ress),
               _ptr(ress2), int(bool(adaptive)), int(min_neval), float(init_calls), float(beta),
               _ptr(ws.rows), ws.stride, int(n_rows), int(bool(train)), int(bool(refine)),
-              _ptr(grid), _ptr(out), _ptr(arr_var_out), _ptr(hist_out), _stream())
+              _ptr(grid), _ptr(out), _ptr(arr_var_out), _ptr(hist_out),
+              None if counter is None else counter.draw.data_ptr(),
+              None if counter is None else counter.history.data_ptr(),
+              0 if counter is None else counter.capacity, _stream())
     return out[:3]
 
 

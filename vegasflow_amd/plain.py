@@ -45,6 +45,19 @@ class PlainFlow(MonteCarloFlow):
         sigma = torch.sqrt(torch.clamp(err_tmp2, min=0.0))
         return res, sigma
 
+    def _graph_key(self):
+        xmin, xdelta, _ = self._limits()
+        ws = self._workspace_for(kernels.partial_stride(self.n_dim, 1, False))
+        return ("plain", self._compile_serial, self.n_events, self.precision, self._seed, ws.rows.data_ptr(),
+                None if xmin is None else xmin.data_ptr(), None if xdelta is None else xdelta.data_ptr())
+
+    def _graph_iteration(self, counter):
+        n = self.n_events
+        ws = self._workspace_for(kernels.partial_stride(self.n_dim, 1, False))
+        step = self._step(n, train=False, offset=0, draw_dev=counter.draw)
+        ng = kernels.plain_fused(step, self._fused, ws)
+        kernels.vegas_finalize(self.n_dim, 1, False, False, n, ws, ng, counter=counter)
+
     def _fused_iteration(self, inject=None):
         self._next_draw()
         n = self.n_events

@@ -208,6 +208,30 @@ class VegasFlowPlus(VegasFlow):
             self.refine_grid(arr_res2)
         return res, sigma
 
+    def _graph_key(self):
+        xmin, xdelta, _ = self._limits()
+        ws = self._workspace_for(kernels.partial_stride(self.n_dim, 1, self.train))
+        return ("vplus", self._compile_serial, self.train, self._adaptive, self._events_bound(), self.precision,
+                self._seed, self._n_strat, self._n_cubes, self.divisions.data_ptr(), self._n_ev.data_ptr(),
+                self._offsets.data_ptr(), self._ress.data_ptr(), self._ress2.data_ptr(), ws.rows.data_ptr(),
+                None if xmin is None else xmin.data_ptr(), None if xdelta is None else xdelta.data_ptr())
+
+    def _graph_iteration(self, counter):
+        ws = self._workspace_for(kernels.partial_stride(self.n_dim, 1, self.train))
+        step = self._step(self._events_bound(), train=self.train, offset=0, xjac=self._modified_jac,
+                          draw_dev=counter.draw)
+        strata = self._strata()
+        ng = kernels.vplus_fused(step, strata, self._fused, self.divisions, self._ress, self._ress2, ws)
+        return kernels.vplus_finalize(self.n_dim, strata, self._n_ev, self._offsets, self._ress, self._ress2,
+                                      self._adaptive, self.min_neval_hcube, self._init_calls, BETA, ws, ng,
+                                      self.train, self.train, self.divisions, counter=counter)
+
+    def _replayed_events(self, events):
+        # events of each replayed iteration, and the next iteration's count
+        # from the last finalize
+        self._pending_events.extend(int(e) for e in events)
+        self._last_out = self._replay_last
+
     def _fused_iteration(self, inject=None):
         self._next_draw()
         ws = self._workspace_for(kernels.partial_stride(self.n_dim, 1, self.train))
