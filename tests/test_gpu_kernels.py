@@ -282,8 +282,10 @@ def test_fused_large_event_count_multi_block():
 
 
 # ------------------------------------------------------- unfused digest
-@pytest.mark.parametrize("d", [3, 8, 20])
+@pytest.mark.parametrize("d", [3, 8, 12, 14, 20])
 def test_sample_digest_path_matches_oracle(d):
+    """Digest through pair tables read four bins per load (d = 8, 12), one
+    bin per load (3, 14), and per-dimension histograms (20)."""
     n = 10007
     grid = random_grid(np.random.default_rng(40 + d), d)
     step = kernels.make_step(d, n, precision="f64", train=True, seed=2, draw_id=6, xjac=1.0 / n)

@@ -488,6 +488,53 @@ __global__ __launch_bounds__(kBlock) void vegas_digest(StepK s, const FT* __rest
     }
   }
 }
+
+// Training digest of a scalar integrand with pair histograms (fused_event.hpp)
+// for 2 <= n_dim <= kPairMaxDim: one LDS add per pair of dimensions into a
+// workgroup's pair tables (LDS: pair_cells(d) doubles), so the kernel is
+// bound by its HBM reads (f, w, bins) rather than by LDS atomics. Rows of
+// bins whose length is a multiple of 4 are read four bins per load. Bins
+// are clamped to 0..49 as in vegas_digest.
+template <typename FT, int BS>
+__global__ __launch_bounds__(BS) void vegas_digest_pairs(StepK s, const FT* __restrict__ f,
+                                                         int64_t fstride, const FT* __restrict__ w,
+                                                         int64_t wstride, const uint8_t* __restrict__ idx,
+                                                         RowsK R) {
+  extern __shared__ double H[];
+  const int d = s.n_dim;
+  const int half = pair_cells(d) / 2;
+  for (int i = threadIdx.x; i < half; i += BS) reinterpret_cast<double2*>(H)[i] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int np = (d + 1) >> 1;
+  const bool words = (d & 3) == 0 && ((uintptr_t)idx & 3u) == 0;
+  const uint32_t top = kBins - 1;
+  double acc = 0.0, acc2 = 0.0;
+  const int64_t step = (int64_t)gridDim.x * BS;
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < s.n_events; i += step) {
+    const double tmp = (double)w[i * wstride] * (double)f[i * fstride];  // vflow.py:416
+    const double tmp2 = tmp * tmp;
+    acc += tmp;
+    acc2 += tmp2;
+    const uint8_t* ir = idx + i * d;
+    if (words) {
+      const uint32_t* ir4 = reinterpret_cast<const uint32_t*>(ir);  // 4-aligned: idx is, d % 4 == 0
+      for (int g = 0; g < (d >> 2); ++g) {
+        const uint32_t v = ir4[g];
+        const uint32_t b0 = min(v & 0xffu, top), b1 = min((v >> 8) & 0xffu, top);
+        const uint32_t b2 = min((v >> 16) & 0xffu, top), b3 = min(v >> 24, top);
+        lds_add(&H[(2 * g) * kPairBins + b0 * kBins + b1], tmp2);
+        lds_add(&H[(2 * g + 1) * kPairBins + b2 * kBins + b3], tmp2);
+      }
+    } else {
+      for (int p = 0; p < np; ++p) {
+        const uint32_t b0 = min((uint32_t)ir[2 * p], top);
+        const uint32_t b1 = 2 * p + 1 < d ? min((uint32_t)ir[2 * p + 1], top) : 0u;
+        lds_add(&H[p * kPairBins + b0 * kBins + b1], tmp2);
+      }
+    }
+  }
+  finish_pairs<BS>(acc, acc2, H, d, group_row(R));
+}
 // ---------------------------------------------------------------------------
 // finalize: block 0 sums and sigma, block 1 + j dimension j (refine.hpp)
 // ---------------------------------------------------------------------------
@@ -752,9 +799,24 @@ extern "C" int mv_vegas_digest(const mv_step* step, const void* f, int64_t f_str
   if ((rc = make_rows(parts, mv_partial_stride(step->n_dim, n_out, step->train),
                       "mv_vegas_digest", &R)))
     return rc;
-  const int blocks = producer_blocks(step->n_events);
   const StepK k = make_stepk(*step);
   hipStream_t st = as_stream(stream);
+  const int d = step->n_dim;
+  if (step->train && n_out == 1 && d >= kPairMinDim && d <= kPairMaxDim) {
+    const size_t lds = (size_t)pair_cells(d) * sizeof(double);
+    if (lds <= max_lds_per_block()) {
+      const int b = step->precision == MV_F64
+          ? launch_producer<kPairBlock>(vegas_digest_pairs<double, kPairBlock>, step->n_events, lds, st, k,
+                                        static_cast<const double*>(f), f_stride,
+                                        static_cast<const double*>(w), w_stride, idx, R)
+          : launch_producer<kPairBlock>(vegas_digest_pairs<float, kPairBlock>, step->n_events, lds, st, k,
+                                        static_cast<const float*>(f), f_stride,
+                                        static_cast<const float*>(w), w_stride, idx, R);
+      if ((rc = check_launch("mv_vegas_digest"))) return rc;
+      return rows_used(b);
+    }
+  }
+  const int blocks = producer_blocks(step->n_events);
   const int slices = step->train ? (step->n_dim + kDigestDims - 1) / kDigestDims : 1;
   if (slices > 65535) return fail(MV_ERR_UNSUPPORTED, "mv_vegas_digest: n_dim %d too large", step->n_dim);
   const dim3 g(blocks, slices);
