@@ -245,22 +245,42 @@ inline size_t stage_bytes(int d, size_t elem, bool bins) {
   return (size_t)kBlock * d * elem + (bins ? (((size_t)kBlock * d + 3) & ~(size_t)3) : 0);
 }
 
+// Copy ``bytes`` bytes from the LDS stage to dst: 16 bytes per thread and
+// store where both sides are 16-byte aligned, then the rest byte-wise by
+// element. ``st`` is 16-byte aligned (x rows at the stage start, bins after
+// kBlock * d elements of 4 or 8 bytes).
+__device__ __forceinline__ void stage_copy(const uint8_t* st, uint8_t* __restrict__ dst, int bytes,
+                                           int elem) {
+  int done = 0;
+  if (((uintptr_t)dst & 15u) == 0 && ((uintptr_t)st & 15u) == 0) {
+    const int vecs = bytes >> 4;
+    const uint4* s16 = reinterpret_cast<const uint4*>(st);
+    uint4* d16 = reinterpret_cast<uint4*>(dst);
+    for (int k = threadIdx.x; k < vecs; k += kBlock) d16[k] = s16[k];
+    done = vecs << 4;
+  }
+  if (elem == 8) {
+    const double* s8 = reinterpret_cast<const double*>(st);
+    double* d8 = reinterpret_cast<double*>(dst);
+    for (int k = (done >> 3) + threadIdx.x; k < (bytes >> 3); k += kBlock) d8[k] = s8[k];
+  } else if (elem == 4) {
+    const float* s4 = reinterpret_cast<const float*>(st);
+    float* d4 = reinterpret_cast<float*>(dst);
+    for (int k = (done >> 2) + threadIdx.x; k < (bytes >> 2); k += kBlock) d4[k] = s4[k];
+  } else {
+    for (int k = done + threadIdx.x; k < bytes; k += kBlock) dst[k] = st[k];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void stage_store(const T* st, T* __restrict__ dst, int elems) {
-  for (int k = threadIdx.x; k < elems; k += kBlock) dst[k] = st[k];
+  stage_copy(reinterpret_cast<const uint8_t*>(st), reinterpret_cast<uint8_t*>(dst),
+             elems * (int)sizeof(T), (int)sizeof(T));
 }
 
 __device__ __forceinline__ void stage_store_bytes(const uint8_t* st, uint8_t* __restrict__ dst,
                                                   int elems) {
-  int done = 0;
-  if (((uintptr_t)dst & 3u) == 0) {  // st is 4-aligned (LDS, after the x rows)
-    const int words = elems >> 2;
-    const uint32_t* s4 = reinterpret_cast<const uint32_t*>(st);
-    uint32_t* d4 = reinterpret_cast<uint32_t*>(dst);
-    for (int k = threadIdx.x; k < words; k += kBlock) d4[k] = s4[k];
-    done = words << 2;
-  }
-  for (int k = done + threadIdx.x; k < elems; k += kBlock) dst[k] = st[k];
+  stage_copy(st, dst, elems, 1);
 }
 
 // vflow.py:87-126 + 33-83, monte_carlo.py:249-275. FP64: the reference's
