@@ -18,6 +18,7 @@ def _pair(cls, *args, **kwargs):
     for graphs in (False, True):
         inst = cls(*args, verbose=False, **kwargs)
         inst.use_graphs = graphs
+        inst.graph_min_iterations = 2  # capture even short runs here
         out.append(inst)
     return out
 
@@ -59,13 +60,29 @@ def test_long_runs_replay_one_graph():
     assert graph._draw_count == eager._draw_count == 26
 
 
+def test_short_first_runs_stay_eager():
+    """Below graph_min_iterations a new shape runs eagerly (a capture would
+    cost more than it saves); a long run captures and later short runs of
+    the same shape replay."""
+    inst = vf.VegasFlow(4, 50_000, seed=2, verbose=False)
+    inst.compile(vf.Gaussian(0.2, 0.5))
+    inst.run_integration(3)
+    assert not inst._graphs
+    inst.run_integration(inst.graph_min_iterations)
+    assert len(inst._graphs) == 1
+    inst.run_integration(2)
+    assert len(inst._graphs) == 1 and inst._draw_count == 5 + inst.graph_min_iterations
+
+
 def test_vegasflow_replay_with_limits_and_single_iterations():
     eager, graph = _pair(vf.VegasFlow, 3, 50_000, seed=3, xmin=[0.0, -1.0, 2.0], xmax=[1.0, 1.0, 3.5])
     fn = vf.Product()
     for inst in (eager, graph):
         inst.compile(fn)
+        inst.run_integration(2)  # captures
         for _ in range(3):
-            inst.run_integration(1)
+            inst.run_integration(1)  # one replay each
+    assert len(graph._graphs) == 1
     _same_history(eager, graph)
 
 

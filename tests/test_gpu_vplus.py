@@ -159,6 +159,50 @@ def test_vplus_digest_path_matches_fused():
     assert np.abs(e1.astype(int) - e2.astype(int)).max() <= 1
 
 
+@pytest.mark.parametrize("pattern", ["one_event_each", "empty_runs"])
+def test_vplus_fused_cube_search_tiny_and_empty_cubes(pattern):
+    """Per-cube sums of the fused kernel (one search per wave, then a
+    lane-parallel search over the next 64 offsets) equal those of the sample
+    + digest path (a whole-array search per event) when a wave spans 64 cubes
+    of one event each, and when runs of empty cubes push events past the
+    64-offset window."""
+    d, n_strat = 3, 12
+    n_cubes = n_strat ** d
+    rng = np.random.default_rng(5)
+    if pattern == "one_event_each":
+        n_ev = np.ones(n_cubes, dtype=np.int32)
+    else:
+        n_ev = rng.integers(0, 3, size=n_cubes).astype(np.int32)
+        n_ev[100:300] = 0
+        n_ev[700:720] = 0
+        n_ev[0] = 1
+    st = {"n_strat": n_strat, "n_cubes": n_cubes, "n_ev": n_ev}
+    total = int(n_ev.sum())
+    grid = random_grid(rng, d)
+    sums = []
+    for path in ("fused", "unfused"):
+        _, _, strata = _device_strata(st)
+        g = dev(grid)
+        ress = torch.zeros(n_cubes, dtype=torch.float64, device=DEV)
+        ress2 = torch.zeros_like(ress)
+        ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, False))
+        step = kernels.make_step(d, total, precision="f64", train=False, seed=4, xjac=1.0 / n_cubes)
+        if path == "fused":
+            kernels.vplus_fused(step, strata, MIX.to_struct(d), g, ress, ress2, ws)
+        else:
+            x = torch.empty((total, d), dtype=torch.float64, device=DEV)
+            w = torch.empty(total, dtype=torch.float64, device=DEV)
+            idx = torch.empty((total, d), dtype=torch.uint8, device=DEV)
+            seg = torch.empty(total, dtype=torch.int32, device=DEV)
+            kernels.vplus_sample(step, strata, g, x, w, idx, seg)
+            kernels.vplus_digest(step, strata, MIX(x), w, idx, seg, ress, ress2, ws)
+        sums.append((ress.cpu().numpy(), ress2.cpu().numpy()))
+    (r1, q1), (r2, q2) = sums
+    np.testing.assert_array_equal(r1 == 0.0, n_ev == 0)
+    np.testing.assert_allclose(r1, r2, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(q1, q2, rtol=1e-12, atol=1e-300)
+
+
 def test_vplus_mixed_tracks_f64():
     d, n = 4, 1_000_000
     st = ora.vplus_setup(d, n, True)

@@ -35,6 +35,37 @@ __device__ __forceinline__ int find_cube(const int32_t* __restrict__ off, int n_
   return lo;
 }
 
+// Cube and n_ev of event e for a wave of 64 consecutive events starting at
+// the wave-uniform e0 (e = e0 + lane, clamped to the last event): one search
+// (uniform) for the cube c0 of e0, then a lane-parallel binary lifting over
+// the next 64 offsets, held one per lane and read with shuffles that every
+// lane executes. Cubes hold at least one event, so the wave's events lie in
+// c0 .. c0 + 63; a lane whose event reaches the window's last cube (possible
+// only with empty cubes past it) searches the whole array instead.
+__device__ __forceinline__ int wave_find_cube(const int32_t* __restrict__ off,
+                                              const int32_t* __restrict__ n_ev, int n_cubes, int e0,
+                                              int e, int& nev) {
+  const int c0 = find_cube(off, n_cubes, e0);
+  const int lane = threadIdx.x & 63;
+  const int lim = min(n_cubes - c0, 64);
+  const int ok = lane < lim ? off[c0 + lane] : 0x7fffffff;
+  const int nk = lane < lim ? n_ev[c0 + lane] : 0;
+  int lo = 0;
+#pragma unroll
+  for (int stp = 32; stp >= 1; stp >>= 1) {
+    const int probe = lo + stp;  // <= 63
+    const int v = __shfl(ok, probe, 64);
+    if (probe < lim && v <= e) lo = probe;
+  }
+  nev = __shfl(nk, lo, 64);
+  int c = c0 + lo;
+  if (lo == 63) {
+    c = find_cube(off, n_cubes, e);
+    nev = n_ev[c];
+  }
+  return c;
+}
+
 // q = c / n, r = c % n for 0 <= c < 2^24 through an FP32 reciprocal and
 // one correction step
 __device__ __forceinline__ void divmod_small(int c, int n, float rn, int& q, int& r) {
@@ -140,12 +171,16 @@ __global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_in
   const int total = st.off[st.n_cubes];
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
-  for (int64_t base = (int64_t)blockIdx.x * kBlock + wave * 64; base < total; base += step) {
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  for (int64_t base = (int64_t)blockIdx.x * kBlock + wave_u * 64; base < total; base += step) {
     const int64_t i = base + lane;
+    int nev;
+    const int wcube = wave_find_cube(st.off, st.n_ev, st.n_cubes, (int)base,
+                                     (int)(i < total ? i : total - 1), nev);
     int cube = -1;
     double tmp = 0.0, tmp2 = 0.0;
     if (i < total) {
-      cube = find_cube(st.off, st.n_cubes, (int)i);
+      cube = wcube;
       int h[MV_FUSED_MAX_DIM];
       cube_digits(cube, d, st.n_strat, rn, h);
       const uint64_t ge = (uint64_t)(s.event_offset + i);
@@ -187,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_in
         }
       }
       // vflowplus.py:77, monte_carlo.py:270-274, vflowplus.py:209
-      const double wev = (double)w / (double)st.n_ev[cube];
+      const double wev = (double)w / (double)nev;
       tmp = (wev * wscale) * (double)F.value();
       tmp2 = tmp * tmp;
       acc += tmp;

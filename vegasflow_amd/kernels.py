@@ -126,21 +126,37 @@ def make_step(n_dim, n_events, *, precision="f64", train=True, event_offset=0,
     return s
 
 
+_capture_streams = {}
+
+
 def capture_graph(body):
-    """Capture ``body()`` (launches on the current stream) as a HIP graph;
-    returns (graph, what body returned). Python's cyclic garbage collector is
-    off while the capture runs: a collection there can destroy an unreachable
-    graph or release device memory, operations a stream capture does not
-    allow (torch collects once, before the capture begins)."""
+    """Capture ``body()`` (launches on the current stream) as a HIP graph on
+    a side stream ordered after the current one; returns (graph, what body
+    returned). Python's cyclic garbage collector is off while the capture
+    runs: a collection there can destroy an unreachable graph or release
+    device memory, operations a stream capture does not allow. (Unlike
+    torch.cuda.graph, no full collection and cache flush precede the
+    capture: they cost milliseconds, more than a short run's replays save.)"""
+    dev = torch.cuda.current_device()
+    side = _capture_streams.get(dev)
+    if side is None:
+        side = _capture_streams[dev] = torch.cuda.Stream(dev)
+    cur = torch.cuda.current_stream()
+    side.wait_stream(cur)
     graph = torch.cuda.CUDAGraph()
     enabled = gc.isenabled()
     gc.disable()
     try:
-        with torch.cuda.graph(graph):
-            out = body()
+        with torch.cuda.stream(side):
+            graph.capture_begin()
+            try:
+                out = body()
+            finally:
+                graph.capture_end()
     finally:
         if enabled:
             gc.enable()
+    cur.wait_stream(side)
     return graph, out
 
 
