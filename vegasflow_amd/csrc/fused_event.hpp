@@ -76,6 +76,7 @@ __device__ __forceinline__ void hist_adds(double* wh, const uint32_t (&pk)[4], i
 constexpr int kPairBins = kBins * kBins;
 constexpr int kPairMinDim = 2;
 constexpr int kPairMaxDim = 14;
+constexpr int kPairBlock = 1024;  // threads of a pair-histogram workgroup
 
 __host__ __device__ constexpr int pair_cells(int d) { return ((d + 1) >> 1) * kPairBins; }
 

@@ -205,6 +205,13 @@ __device__ __forceinline__ void finish_pairs(double acc, double acc2, const doub
   }
 }
 
+// The mixed-precision training producer with pair histograms (pairs.hip):
+// kPairBlock-thread workgroups, ``lds`` bytes of table and pair cells, one
+// kernel per (family, n_dim) for kPairMinDim <= n_dim <= kPairMaxDim.
+// Returns the workgroups launched.
+int launch_vegas_fused_pairs(int family, const StepK& k, const mv_integrand& fn, const double* grid,
+                             const RowsK& R, size_t lds, hipStream_t st);
+
 template <bool TRAIN, bool SUMS = true>
 __device__ __forceinline__ void finish_block(double acc, double acc2, const double* hist,
                                              int hist_len, const RowsK& R) {

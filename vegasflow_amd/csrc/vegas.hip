@@ -137,51 +137,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   finish_block<TRAIN>(acc, acc2, hist, nb, R);
 }
 
-// Mixed-precision training variant with pair histograms (fused_event.hpp)
-// for n_dim <= kPairMaxDim: one LDS add per pair of dimensions, which halves
-// the histogram atomics that bound vegas_fused_mixed. BS-thread workgroups
-// share one set of pair tables; LDS: the map table (nb float2), then
-// pair_cells(d) doubles, about 83 KB at d = 8, so one workgroup per CU. (The
-// FP64 kernel is bound by its VALU work and measured slower at that
-// occupancy, so it keeps per-wave histograms.)
-template <int FAM, int BS>
-__global__ __launch_bounds__(BS) void vegas_fused_pairs(StepK s, mv_integrand fn,
-                                                        const double* __restrict__ grid, RowsK R) {
-  const uint32_t draw = draw_of(s);
-  extern __shared__ double smem[];
-  const int d = s.n_dim;
-  const int nb = d * kBins;
-  float2* tab = reinterpret_cast<float2*>(smem);
-  double* H = smem + nb;  // 16-byte aligned: nb * 8 is a multiple of 400
-  build_table<float, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, nullptr);
-  const int half = pair_cells(d) / 2;
-  for (int i = threadIdx.x; i < half; i += BS) reinterpret_cast<double2*>(H)[i] = make_double2(0.0, 0.0);
-  const float cprod = jacobian_product<FAM>(fn, d, s.xdelta);
-  __syncthreads();
-  Integrand<float, FAM> F(fn);
-  const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
-  const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
-  double acc = 0.0, acc2 = 0.0;
-  const int64_t step = (int64_t)gridDim.x * BS;
-  const int64_t i0 = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (inj) {
-    for (int64_t i = i0; i < s.n_events; i += step) {
-      const double tmp = event_mixed_pairs<FAM, true>(F, 0u, 0u, 0u, s.key, d, tab, cprod, H,
-                                                      inj + i * d, wscale);
-      acc += tmp;
-      acc2 += tmp * tmp;
-    }
-  } else {
-    for (int64_t i = i0; i < s.n_events; i += step) {
-      const double tmp = event_mixed_pairs<FAM, false>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw,
-                                                       s.key, d, tab, cprod, H, nullptr, wscale);
-      acc += tmp;
-      acc2 += tmp * tmp;
-    }
-  }
-  finish_pairs<BS>(acc, acc2, H, d, group_row(R));
-}
-
 // ---------------------------------------------------------------------------
 // fused PlainFlow (plain.py:18-35)
 // ---------------------------------------------------------------------------
@@ -649,8 +604,6 @@ extern "C" int32_t mv_producer_rows(int64_t n_events) {
 
 namespace {
 
-constexpr int kPairBlock = 1024;  // threads of a pair-histogram workgroup
-
 // LDS bytes of vegas_fused_pairs, 0 where it does not apply (FP64, no
 // training, n_dim outside kPairMinDim..kPairMaxDim, or more LDS than a
 // workgroup may have)
@@ -666,8 +619,7 @@ int launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool 
   const int nb = k.n_dim * kBins;
   const int64_t n = k.n_events;
   if (const size_t plds = pairs_lds(k.n_dim, mixed, train))
-    return launch_producer<kPairBlock>(vegas_fused_pairs<FAM, kPairBlock>, n, plds, st, k, fn,
-                                       grid, R);
+    return launch_vegas_fused_pairs(FAM, k, fn, grid, R, plds, st);
   if (!mixed) {
     const size_t lds = (size_t)(3 * nb + (train ? kWaves * nb : 0)) * sizeof(double);
     return train ? launch_producer(vegas_fused_f64<FAM, true>, n, lds, st, k, fn, grid, R)
