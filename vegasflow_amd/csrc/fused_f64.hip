@@ -1,0 +1,89 @@
+// fused_f64.hip - the FP64 fused VegasFlow producer (the drop-in;
+// vflow.py:87-126 + 33-83 + 389-430, monte_carlo.py:249-275), compiled per
+// (integrand family, train, n_dim) so that the event body is straight-line
+// code for the n_dim of the launch.
+//
+// Per event: Philox draws (two words per dimension), bin and fraction in the
+// reference's FP64 operation order, the folded (P, Q, W) table in LDS, the
+// integrand, w f sums; when training, (w f)^2 into the wave's private LDS
+// histogram (kWaves copies of n_dim x 50 doubles). LDS: 3 + 4 (training)
+// doubles per (dimension, bin).
+#include <array>
+#include <utility>
+
+#include "abi_util.hpp"
+#include "device.hpp"
+#include "fused_event.hpp"
+#include "rows.hpp"
+
+namespace mv {
+
+template <int FAM, bool TRAIN, int D>
+__global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand fn,
+                                                          const double* __restrict__ grid, RowsK R) {
+  const uint32_t draw = draw_of(s);
+  extern __shared__ double smem[];
+  constexpr int nb = D * kBins;
+  double2* tab = reinterpret_cast<double2*>(smem);
+  double* tw = smem + 2 * nb;
+  double* hist = tw + nb;
+  build_table<double, FAM>(fn, grid, D, s.xmin, s.xdelta, tab, tw);
+  if (TRAIN)
+    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+  __syncthreads();
+  double* wh = hist + (threadIdx.x >> 6) * nb;
+  Integrand<double, FAM> F(fn);
+  const double* inj = static_cast<const double*>(s.inject);
+  const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
+  double acc = 0.0, acc2 = 0.0;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
+    const double tmp = event_f64<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key,
+                                             (D + 3) / 4, D, tab, tw, wh, inj ? inj + i * D : nullptr,
+                                             wscale);
+    acc += tmp;
+    acc2 += tmp * tmp;
+  }
+  finish_block<TRAIN>(acc, acc2, hist, nb, R);
+}
+
+namespace {
+
+using Kernel = void (*)(StepK, mv_integrand, const double*, RowsK);
+
+template <int FAM, bool TRAIN, int... Ds>
+constexpr std::array<Kernel, sizeof...(Ds)> kernels_of(std::integer_sequence<int, Ds...>) {
+  return {&vegas_fused_f64<FAM, TRAIN, Ds + 1>...};
+}
+
+template <int FAM, bool TRAIN>
+int launch_fam(const StepK& k, const mv_integrand& fn, const double* grid, const RowsK& R,
+               hipStream_t st) {
+  static constexpr auto table =
+      kernels_of<FAM, TRAIN>(std::make_integer_sequence<int, MV_FUSED_MAX_DIM>{});
+  if (k.n_dim < 1 || k.n_dim > MV_FUSED_MAX_DIM)
+    return fail(MV_ERR_UNSUPPORTED, "fused FP64 kernel: n_dim %d outside 1..%d", k.n_dim,
+                MV_FUSED_MAX_DIM);
+  const int nb = k.n_dim * kBins;
+  const size_t lds = (size_t)(3 * nb + (TRAIN ? kWaves * nb : 0)) * sizeof(double);
+  return launch_producer(table[k.n_dim - 1], k.n_events, lds, st, k, fn, grid, R);
+}
+
+}  // namespace
+
+int launch_vegas_fused_f64(int family, bool train, const StepK& k, const mv_integrand& fn,
+                           const double* grid, const RowsK& R, hipStream_t st) {
+#define MV_F64_FAM(F) \
+  return train ? launch_fam<F, true>(k, fn, grid, R, st) : launch_fam<F, false>(k, fn, grid, R, st)
+  switch (family) {
+    case MV_FN_CONSTANT: MV_F64_FAM(MV_FN_CONSTANT);
+    case MV_FN_GAUSSIAN: MV_F64_FAM(MV_FN_GAUSSIAN);
+    case MV_FN_PRODUCT: MV_F64_FAM(MV_FN_PRODUCT);
+    case MV_FN_GAUSSMIX: MV_F64_FAM(MV_FN_GAUSSMIX);
+    case MV_FN_BWPOWER: MV_F64_FAM(MV_FN_BWPOWER);
+    default: return fail(MV_ERR_UNSUPPORTED, "fused FP64 kernel: unknown family %d", family);
+  }
+#undef MV_F64_FAM
+}
+
+}  // namespace mv

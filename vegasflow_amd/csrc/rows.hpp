@@ -205,6 +205,14 @@ __device__ __forceinline__ void finish_pairs(double acc, double acc2, const doub
   }
 }
 
+// The fused producers with per-wave histograms (fused.hip), compiled per
+// (family, train, n_dim) for 1 <= n_dim <= MV_FUSED_MAX_DIM: FP64 and mixed
+// precision. Return the workgroups launched.
+int launch_vegas_fused_f64(int family, bool train, const StepK& k, const mv_integrand& fn,
+                           const double* grid, const RowsK& R, hipStream_t st);
+int launch_vegas_fused_mixed(int family, bool train, const StepK& k, const mv_integrand& fn,
+                             const double* grid, const RowsK& R, hipStream_t st);
+
 // The mixed-precision training producer with pair histograms (pairs.hip):
 // kPairBlock-thread workgroups, ``lds`` bytes of table and pair cells, one
 // kernel per (family, n_dim) for kPairMinDim <= n_dim <= kPairMaxDim.

@@ -69,75 +69,6 @@ size_t max_lds_per_block() {
 }
 
 // ---------------------------------------------------------------------------
-// fused VegasFlow (vflow.py:87-126 + 33-83 + 389-430, monte_carlo.py:249-275)
-// LDS: the map table of fused_event.hpp and kWaves private histograms.
-// ---------------------------------------------------------------------------
-template <int FAM, bool TRAIN>
-__global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand fn,
-                                                          const double* __restrict__ grid,
-                                                          RowsK R) {
-  const uint32_t draw = draw_of(s);
-  extern __shared__ double smem[];
-  const int d = s.n_dim;
-  const int nb = d * kBins;
-  double2* tab = reinterpret_cast<double2*>(smem);
-  double* tw = smem + 2 * nb;
-  double* hist = tw + nb;
-  build_table<double, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, tw);
-  if (TRAIN)
-    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
-  __syncthreads();
-  double* wh = hist + (threadIdx.x >> 6) * nb;
-  Integrand<double, FAM> F(fn);
-  const double* inj = static_cast<const double*>(s.inject);
-  const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
-  double acc = 0.0, acc2 = 0.0;
-  const int64_t step = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const double tmp = event_f64<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key,
-                                             s.groups, d, tab, tw, wh, inj ? inj + i * d : nullptr,
-                                             wscale);
-    acc += tmp;
-    acc2 += tmp * tmp;
-  }
-  finish_block<TRAIN>(acc, acc2, hist, nb, R);
-}
-
-template <int FAM, bool TRAIN>
-// Eight waves per SIMD (the compiler keeps the SGPR count low enough): the
-// training kernel is bound by the LDS histogram atomics, and more resident
-// waves overlap them better with the Philox VALU work.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void vegas_fused_mixed(StepK s, mv_integrand fn,
-                                                            const double* __restrict__ grid,
-                                                            RowsK R) {
-  const uint32_t draw = draw_of(s);
-  extern __shared__ double smem[];
-  const int d = s.n_dim;
-  const int nb = d * kBins;
-  float2* tab = reinterpret_cast<float2*>(smem);  // nb float2
-  double* hist = smem + nb;
-  build_table<float, FAM>(fn, grid, d, s.xmin, s.xdelta, tab, nullptr);
-  if (TRAIN)
-    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
-  const float cprod = jacobian_product<FAM>(fn, d, s.xdelta);
-  __syncthreads();
-  double* wh = hist + (threadIdx.x >> 6) * nb;
-  Integrand<float, FAM> F(fn);
-  const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
-  const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;
-  double acc = 0.0, acc2 = 0.0;
-  const int64_t step = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const double tmp = event_mixed<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key,
-                                               s.groups, d, tab, cprod, wh, inj ? inj + i * d : nullptr,
-                                               wscale);
-    acc += tmp;
-    acc2 += tmp * tmp;
-  }
-  finish_block<TRAIN>(acc, acc2, hist, nb, R);
-}
-
-// ---------------------------------------------------------------------------
 // fused PlainFlow (plain.py:18-35)
 // ---------------------------------------------------------------------------
 template <int FAM, bool MIXED>
@@ -616,18 +547,10 @@ size_t pairs_lds(int d, bool mixed, bool train) {
 template <int FAM>
 int launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool train,
                        const double* grid, const RowsK& R, hipStream_t st) {
-  const int nb = k.n_dim * kBins;
-  const int64_t n = k.n_events;
   if (const size_t plds = pairs_lds(k.n_dim, mixed, train))
     return launch_vegas_fused_pairs(FAM, k, fn, grid, R, plds, st);
-  if (!mixed) {
-    const size_t lds = (size_t)(3 * nb + (train ? kWaves * nb : 0)) * sizeof(double);
-    return train ? launch_producer(vegas_fused_f64<FAM, true>, n, lds, st, k, fn, grid, R)
-                 : launch_producer(vegas_fused_f64<FAM, false>, n, lds, st, k, fn, grid, R);
-  }
-  const size_t lds = (size_t)(nb + (train ? kWaves * nb : 0)) * sizeof(double);
-  return train ? launch_producer(vegas_fused_mixed<FAM, true>, n, lds, st, k, fn, grid, R)
-               : launch_producer(vegas_fused_mixed<FAM, false>, n, lds, st, k, fn, grid, R);
+  return mixed ? launch_vegas_fused_mixed(FAM, train, k, fn, grid, R, st)
+               : launch_vegas_fused_f64(FAM, train, k, fn, grid, R, st);
 }
 
 template <int FAM>
