@@ -10,11 +10,15 @@ namespace mv {
 // Lanes 0..49 own a bin, lanes 0..50 an edge. The reference's 49 serial cuts
 // (vflow.py:194-207) are computed independently, lane k making cut k: the
 // serial loop stops in the first bin n whose running weight reaches k * ave,
-// with bw = cumsum(wei)[n] - k * ave left over, so lane k finds n by
-// counting the prefix sums below k * ave and interpolates as the reference
-// does, prev = 0.0 (not div[0]) for n = 0 included. Only the rounding of bw
-// differs from the serial accumulation (a few ulp of sum(wei)). n is capped
-// at the last bin; the reference has no guard there.
+// with bw = cumsum(wei)[n] - k * ave left over, so lane k finds n, the count
+// of prefix sums below k * ave (a binary search over the non-decreasing
+// prefix sums, held one per lane), and interpolates as the reference does,
+// prev = 0.0 (not div[0]) for n = 0 included. Only the rounding of bw differs
+// from the serial accumulation (a few ulp of sum(wei)). n is capped at the
+// last bin; the reference has no guard there. The wave runs alone on its CU,
+// so its instruction latency is the kernel's time: the weight
+// (1 - r)/(log(sum) - log(s))^1.5 is taken as aux * sqrt(aux) with one log of
+// sum / s, equal to the reference's pow and two logs to a few ulp.
 __device__ inline void refine_wave(const double* h /* LDS, 50 */, double* __restrict__ div /* 51 */) {
   const int lane = threadIdx.x & 63;
   const double edge = lane <= kBins ? div[lane] : 0.0;
@@ -30,8 +34,8 @@ __device__ inline void refine_wave(const double* h /* LDS, 50 */, double* __rest
   const double sum_t = bcast_lane(wave_sum(s), 0);            // vflow.py:162
   double wei = 0.0;
   if (lane < kBins) {
-    const double aux = (1.0 - s / sum_t) / (log(sum_t) - log(s));  // vflow.py:163-164
-    wei = pow(aux, 1.5);                                           // vflow.py:165 (ALPHA)
+    const double aux = (1.0 - s / sum_t) / log(sum_t / s);  // vflow.py:163-164
+    wei = aux * sqrt(aux);                                   // vflow.py:165, ALPHA = 1.5
   }
   const double ave = bcast_lane(wave_sum(wei), 0) / (double)kBins;  // vflow.py:166
 
@@ -43,8 +47,15 @@ __device__ inline void refine_wave(const double* h /* LDS, 50 */, double* __rest
     if (lane >= o) c += t;
   }
   const double target = (double)lane * ave;  // cut k = lane
+  // n = #{b < 50 : c_b < target}: c is non-decreasing over lanes 0..63 (lanes
+  // >= 50 repeat the total), so binary lifting over lanes; every lane runs
+  // every shuffle
   int n = 0;
-  for (int b = 0; b < kBins; ++b) n += bcast_lane(c, b) < target ? 1 : 0;
+#pragma unroll
+  for (int stp = 32; stp >= 1; stp >>= 1) {
+    const double v = __shfl(c, n + stp - 1, 64);  // c_(n + stp - 1), index <= 62
+    if (n + stp <= kBins && v < target) n += stp;
+  }
   if (n > kBins - 1) n = kBins - 1;
   // every shuffle runs on the whole wave (a lane-gather from an inactive
   // lane reads 0); the n == 0 case is selected afterwards
