@@ -34,12 +34,12 @@ PEAK = {"mixed": 157.3, "f64": 78.6}  # TFLOP/s, MI355X datasheet (FP32 / FP64 v
 # FETCH_SIZE and WRITE_SIZE passes of rocprofv3 scaled by a 1 GiB copy
 # (profiles/vegas_fused_gauss_d8_1e7_pmc.md): 0 bytes per event; the grid
 # reads and the FP64 atomic adds of the partial rows
-TRAFFIC = {"mixed": 1.0025e6, "f64": 5.3197e6}
+TRAFFIC = {"mixed": 0.9748e6, "f64": 3.5584e6}
 TRAFFIC_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_pmc.md"
 # VALU wave instructions per launch (SQ_INSTS_VALU, same profile) and the
 # issue peak they are priced against: one wave64 VALU instruction per SIMD
 # every 4 cycles, 4 SIMDs per CU, 2.4 GHz engine clock (MI355X datasheet)
-VALU_INSTS = {"mixed": 3.099e7, "f64": 6.379e7}
+VALU_INSTS = {"mixed": 2.535e7, "f64": 5.950e7}
 CLOCK_HZ = 2.4e9
 
 
@@ -422,7 +422,7 @@ def main():
         "errors": err,
         "roofline": {
             "bound": "valu",
-            "kernel": "vegas_fused_pairs<GAUSSIAN, 1024> (mixed training)",
+            "kernel": "vegas_fused_pairs<GAUSSIAN, 8> (mixed training, n_dim 8)",
             "achieved": achieved,
             "peak": PEAK["mixed"],
             "unit": "TFLOP/s",
@@ -434,7 +434,7 @@ def main():
             "flop_per_event": flops_per_event(D, True),
             "valu_issue": valu_issue("mixed"),
             "f64_kernel": {
-                "kernel": "vegas_fused_f64<GAUSSIAN, train>",
+                "kernel": "vegas_fused_f64<GAUSSIAN, train, 8>",
                 "achieved": flop / (results["f64"]["fused_kernel_ms"] * 1e-3) / 1e12,
                 "peak": PEAK["f64"],
                 "traffic": TRAFFIC["f64"],
