@@ -108,11 +108,11 @@ __global__ __launch_bounds__(kBlock) void batch_fused(const mv_batch_item* __res
 // LDS = the map table (nbmax float2), then pair_cells(max_dim) doubles.
 // Training items add one LDS atomic per pair of dimensions; frozen items
 // run the plain mixed event body.
-template <int FAM, int BS>
+template <int FAM, int BS, int D = 0>
 __device__ __forceinline__ void batch_tile_pairs(const mv_batch_item& it, const mv_batch_tile& tile,
                                                  uint32_t draw, const double* __restrict__ grids,
                                                  double* __restrict__ rows, double* smem, int nbmax) {
-  const int d = it.n_dim;
+  const int d = D ? D : it.n_dim;  // D > 0: compiled for that n_dim (straight-line event body)
   float2* tab = reinterpret_cast<float2*>(smem);
   double* H = smem + nbmax;  // 16-byte aligned: nbmax * 8 is a multiple of 400
   __syncthreads();            // the previous tile is done with the LDS
@@ -162,7 +162,15 @@ __global__ __launch_bounds__(BS) void batch_fused_pairs(const mv_batch_item* __r
     const mv_batch_item& it = items[tile.item];
     const uint32_t draw = (uint32_t)draws[tile.item];
     if constexpr (FAM >= 0) {
-      batch_tile_pairs<FAM, BS>(it, tile, draw, grids, rows, smem, nbmax);
+      // one event body per n_dim 1..kPairMaxDim, chosen per tile
+#define MV_BTP(D) case D: batch_tile_pairs<FAM, BS, D>(it, tile, draw, grids, rows, smem, nbmax); break
+      switch (it.n_dim) {
+        MV_BTP(1); MV_BTP(2); MV_BTP(3); MV_BTP(4); MV_BTP(5); MV_BTP(6); MV_BTP(7);
+        MV_BTP(8); MV_BTP(9); MV_BTP(10); MV_BTP(11); MV_BTP(12); MV_BTP(13); MV_BTP(14);
+        default: batch_tile_pairs<FAM, BS>(it, tile, draw, grids, rows, smem, nbmax); break;
+      }
+#undef MV_BTP
+      static_assert(kPairMaxDim == 14, "the switch above covers n_dim 1..14");
     } else {
       switch (it.fn.family) {
         case MV_FN_CONSTANT: batch_tile_pairs<MV_FN_CONSTANT, BS>(it, tile, draw, grids, rows, smem, nbmax); break;
