@@ -367,9 +367,11 @@ def test_consume_array_into_indices_kernel():
 
 
 # ----------------------------------------------------------------- plain
-@pytest.mark.parametrize("fam", ["gauss", "product", "bw"])
-def test_plain_fused_matches_oracle(fam):
-    d, n = 4, 50021
+@pytest.mark.parametrize("fam,d", [("gauss", 4), ("gauss", 1), ("gauss", 13), ("product", 4),
+                                   ("product", 20), ("bw", 4), ("bw", 16)])
+def test_plain_fused_matches_oracle(fam, d):
+    """Kernels compiled for n_dim 1..16, and the runtime-n_dim one (d = 20)."""
+    n = 50021
     fn, ofn = FAMILIES[fam]
     step = kernels.make_step(d, n, precision="f64", train=False, seed=21, draw_id=2, xjac=1.0 / n)
     ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, False))

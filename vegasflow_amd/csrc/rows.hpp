@@ -213,6 +213,11 @@ int launch_vegas_fused_f64(int family, bool train, const StepK& k, const mv_inte
 int launch_vegas_fused_mixed(int family, bool train, const StepK& k, const mv_integrand& fn,
                              const double* grid, const RowsK& R, hipStream_t st);
 
+// The fused PlainFlow producer (plain_fused.hip), compiled per (family,
+// precision, n_dim) for 1 <= n_dim <= MV_FUSED_MAX_DIM.
+int launch_plain_fused_dims(int family, bool mixed, const StepK& k, const mv_integrand& fn,
+                            const RowsK& R, hipStream_t st);
+
 // The mixed-precision training producer with pair histograms (pairs.hip):
 // kPairBlock-thread workgroups, ``lds`` bytes of table and pair cells, one
 // kernel per (family, n_dim) for kPairMinDim <= n_dim <= kPairMaxDim.

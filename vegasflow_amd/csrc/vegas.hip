@@ -69,55 +69,6 @@ size_t max_lds_per_block() {
 }
 
 // ---------------------------------------------------------------------------
-// fused PlainFlow (plain.py:18-35)
-// ---------------------------------------------------------------------------
-template <int FAM, bool MIXED>
-__global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, RowsK R) {
-  const uint32_t draw = draw_of(s);
-  using T = typename std::conditional<MIXED, float, double>::type;
-  Integrand<T, FAM> F(fn);
-  const bool limits = s.xmin != nullptr;
-  const double wt = limits ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
-  double acc = 0.0, acc2 = 0.0;
-  const int64_t step = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
-    F.begin();
-    for (int g = 0; g < s.groups; ++g) {
-      uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
-      if (!s.inject) {
-        h = philox(c0, c1, draw, (uint32_t)g, s.key);
-        if (!MIXED) l = philox(c0, c1, draw, (uint32_t)(s.groups + g), s.key);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int j = 4 * g + q;
-        if (j < s.n_dim) {
-          T x;
-          if (MIXED) {
-            const uint32_t hw = s.inject ? static_cast<const uint32_t*>(s.inject)[i * s.n_dim + j]
-                                         : word(h, q);
-            float xf = fmaf((float)(hw >> 9), 0x1p-23f, 0x1p-24f);
-            if (limits) xf = fmaf(xf, (float)s.xdelta[j], (float)s.xmin[j]);
-            x = (T)xf;
-          } else {
-            const double u = s.inject ? static_cast<const double*>(s.inject)[i * s.n_dim + j]
-                                      : draw_f64(word(h, q), word(l, q));
-            x = (T)(limits ? s.xmin[j] + u * s.xdelta[j] : u);
-          }
-          F.add(j, x);
-        }
-      }
-    }
-    const double tmp = (double)F.value() * wt;  // plain.py:27
-    const double tmp2 = tmp * tmp;
-    acc += tmp;
-    acc2 += tmp2;
-  }
-  finish_block<false>(acc, acc2, nullptr, 0, R);
-}
-
-// ---------------------------------------------------------------------------
 // unfused path: sample (any n_dim)
 // With STAGED, each block writes its 256 events' rows of x (and bins) to LDS
 // first and then stores the block's contiguous part of the row-major (n, d)
@@ -556,8 +507,7 @@ int launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool 
 template <int FAM>
 int launch_plain_fused(const StepK& k, const mv_integrand& fn, bool mixed, const RowsK& R,
                        hipStream_t st) {
-  return mixed ? launch_producer(plain_fused<FAM, true>, k.n_events, 0, st, k, fn, R)
-               : launch_producer(plain_fused<FAM, false>, k.n_events, 0, st, k, fn, R);
+  return launch_plain_fused_dims(FAM, mixed, k, fn, R, st);
 }
 
 int sample_blocks(int64_t n) {
