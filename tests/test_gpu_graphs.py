@@ -112,3 +112,15 @@ def test_vegasflowplus_replayed_iterations_match_eager(adaptive):
     np.testing.assert_array_equal(graph._n_ev.cpu().numpy(), eager._n_ev.cpu().numpy())
     np.testing.assert_allclose(graph.divisions.cpu().numpy(), eager.divisions.cpu().numpy(),
                                rtol=1e-10, atol=1e-13)
+
+
+def test_stream_handle_follows_torch_current_stream():
+    """The kernels launch on torch's current stream, also inside a stream
+    context (as during a graph capture)."""
+    from vegasflow_amd import kernels
+
+    assert (kernels._stream().value or 0) == torch.cuda.current_stream().cuda_stream
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        assert (kernels._stream().value or 0) == side.cuda_stream
+    assert (kernels._stream().value or 0) == torch.cuda.current_stream().cuda_stream

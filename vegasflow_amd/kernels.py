@@ -16,7 +16,16 @@ from . import _abi
 BINS = _abi.BINS
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_raw_device = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def _stream():
+    """torch's current HIP stream of the current device, as a raw handle.
+    torch.cuda.current_stream() spends ~10 us per call on device-index
+    bookkeeping; the two C calls below give the same handle directly."""
+    if _raw_stream is not None and _raw_device is not None:
+        return ctypes.c_void_p(_raw_stream(_raw_device()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
@@ -35,12 +44,21 @@ def _need_device(t, name, dtype=None):
         raise ValueError(f"{name} must be contiguous")
 
 
+_gpu_seen = False
+
+
 def require_gpu():
+    """Raise unless a ROCm GPU is available (checked until one is seen; the
+    check costs microseconds and runs once per iteration)."""
+    global _gpu_seen
+    if _gpu_seen:
+        return
     if not torch.cuda.is_available():
         raise RuntimeError(
             "vegasflow_amd runs its kernels on a ROCm GPU and none is available "
             "(torch.cuda.is_available() is False)"
         )
+    _gpu_seen = True
 
 
 def precision_code(precision):
