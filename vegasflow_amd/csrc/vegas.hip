@@ -352,6 +352,8 @@ __global__ __launch_bounds__(kBlock) void vegas_digest(StepK s, const FT* __rest
 // bound by its HBM reads (f, w, bins) rather than by LDS atomics. Rows of
 // bins whose length is a multiple of 4 are read four bins per load. Bins
 // are clamped to 0..49 as in vegas_digest.
+constexpr int kDigestU = 4;  // events per trip of the pair digest's word path
+
 template <typename FT, int BS>
 __global__ __launch_bounds__(BS) void vegas_digest_pairs(StepK s, const FT* __restrict__ f,
                                                          int64_t fstride, const FT* __restrict__ w,
@@ -367,22 +369,58 @@ __global__ __launch_bounds__(BS) void vegas_digest_pairs(StepK s, const FT* __re
   const uint32_t top = kBins - 1;
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * BS;
-  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < s.n_events; i += step) {
-    const double tmp = (double)w[i * wstride] * (double)f[i * fstride];  // vflow.py:416
-    const double tmp2 = tmp * tmp;
-    acc += tmp;
-    acc2 += tmp2;
-    const uint8_t* ir = idx + i * d;
-    if (words) {
-      const uint32_t* ir4 = reinterpret_cast<const uint32_t*>(ir);  // 4-aligned: idx is, d % 4 == 0
-      for (int g = 0; g < (d >> 2); ++g) {
-        const uint32_t v = ir4[g];
-        const uint32_t b0 = min(v & 0xffu, top), b1 = min((v >> 8) & 0xffu, top);
-        const uint32_t b2 = min((v >> 16) & 0xffu, top), b3 = min(v >> 24, top);
-        lds_add(&H[(2 * g) * kPairBins + b0 * kBins + b1], tmp2);
-        lds_add(&H[(2 * g + 1) * kPairBins + b2 * kBins + b3], tmp2);
+  const int64_t i0 = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (words) {
+    // kDigestU events per trip, all their loads issued before the first is
+    // used: with one workgroup per CU the loop is otherwise bound by one
+    // memory latency per event. Sums are added in event order.
+    constexpr int U = kDigestU;
+    constexpr int MAXW = kPairMaxDim / 4;
+    const int nw = d >> 2;
+    for (int64_t i = i0; i < s.n_events; i += U * step) {
+      double tmp[U];
+      uint32_t v[U][MAXW];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t e = i + u * step;
+        ok[u] = e < s.n_events;
+        tmp[u] = 0.0;
+#pragma unroll
+        for (int g = 0; g < MAXW; ++g) v[u][g] = 0u;
+        if (ok[u]) {
+          tmp[u] = (double)w[e * wstride] * (double)f[e * fstride];  // vflow.py:416
+          const uint32_t* ir4 = reinterpret_cast<const uint32_t*>(idx + e * d);  // 4-aligned
+#pragma unroll
+          for (int g = 0; g < MAXW; ++g)
+            if (g < nw) v[u][g] = ir4[g];
+        }
       }
-    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        const double tmp2 = tmp[u] * tmp[u];
+        acc += tmp[u];
+        acc2 += tmp2;
+#pragma unroll
+        for (int g = 0; g < MAXW; ++g) {
+          if (g < nw) {
+            const uint32_t x = v[u][g];
+            const uint32_t b0 = min(x & 0xffu, top), b1 = min((x >> 8) & 0xffu, top);
+            const uint32_t b2 = min((x >> 16) & 0xffu, top), b3 = min(x >> 24, top);
+            lds_add(&H[(2 * g) * kPairBins + b0 * kBins + b1], tmp2);
+            lds_add(&H[(2 * g + 1) * kPairBins + b2 * kBins + b3], tmp2);
+          }
+        }
+      }
+    }
+  } else {
+    for (int64_t i = i0; i < s.n_events; i += step) {
+      const double tmp = (double)w[i * wstride] * (double)f[i * fstride];  // vflow.py:416
+      const double tmp2 = tmp * tmp;
+      acc += tmp;
+      acc2 += tmp2;
+      const uint8_t* ir = idx + i * d;
       for (int p = 0; p < np; ++p) {
         const uint32_t b0 = min((uint32_t)ir[2 * p], top);
         const uint32_t b1 = 2 * p + 1 < d ? min((uint32_t)ir[2 * p + 1], top) : 0u;
