@@ -355,6 +355,7 @@ def main():
     if args.only:
         if args.only == "unfused":
             out = {p: bench_unfused(vf, dt_) for p, dt_ in precisions}
+            out["measured_copy_GB_s"] = calibrate_copy(torch)["GB_per_s_read_plus_write"]
         elif args.only == "vplus":
             out = {p: bench_vplus(vf, dt_) for p, dt_ in precisions}
         elif args.only == "batch":
@@ -446,6 +447,9 @@ def main():
     if not args.no_extra:
         line["unfused_torch_integrand"] = {p: bench_unfused(vf, dt_) for p, dt_ in
                                            (("mixed", torch.float32), ("f64", torch.float64))}
+        # the achievable HBM rate next to the 8 TB/s datasheet figure
+        line["unfused_torch_integrand"]["measured_copy_GB_s"] = \
+            calibrate_copy(torch)["GB_per_s_read_plus_write"]
         line["vegasflowplus_d4"] = {p: bench_vplus(vf, dt_) for p, dt_ in
                                     (("mixed", torch.float32), ("f64", torch.float64))}
         line["batched_256"] = {p: bench_batched(torch, dt_) for p, dt_ in
