@@ -6,7 +6,7 @@ the reference algorithm of vegasflow 1.4.0 for the sample -> map -> integrand
 can be checked on identical inputs.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import it, and only as the checker or as the timed CPU
-baseline. The product path (``vegasflow_amd``) never imports it and has no
+``bench.py`` may import it, and only as the parity reference or as the
+timed CPU baseline. The product path (``vegasflow_amd``) never imports it and has no
 CPU fallback.
 """

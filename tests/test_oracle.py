@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/) before it is trusted as the parity reference.
 
 Each part of the restatement is checked against something it does not
 contain: published known-answer vectors (Philox), an independent
@@ -192,7 +192,7 @@ def test_gaussian_integral_within_errors():
     d, n = 4, 100_000
     hist, _ = ora.run_vegas(lambda x: ora.gaussian(x, 0.1, 0.5), d, n, 5, seed=7)
     res, sig = ora.weighted_average(hist[2:])
-    exact = math.erf(5.0) ** 4  # 0.99999999999385025 (SURVEY closed form)
+    exact = math.erf(5.0) ** 4  # 0.99999999999385025 (closed form)
     assert abs(exact - 0.99999999999385025) < 1e-15
     assert abs(res - exact) < 4 * sig
 
