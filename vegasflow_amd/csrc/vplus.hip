@@ -14,225 +14,9 @@
 #include "fused_event.hpp"
 #include "refine.hpp"
 #include "rows.hpp"
+#include "vplus.hpp"
 
 namespace mv {
-
-struct StrataK {
-  int n_strat;
-  int n_cubes;
-  const int32_t* n_ev;
-  const int32_t* off;
-};
-
-// largest c with off[c] <= e (off strictly increasing, off[0] = 0)
-__device__ __forceinline__ int find_cube(const int32_t* __restrict__ off, int n_cubes, int e) {
-  int lo = 0, hi = n_cubes;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= e) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// Cube and n_ev of event e for a wave of 64 consecutive events starting at
-// the wave-uniform e0 (e = e0 + lane, clamped to the last event): one search
-// (uniform) for the cube c0 of e0, then a lane-parallel binary lifting over
-// the next 64 offsets, held one per lane and read with shuffles that every
-// lane executes. Cubes hold at least one event, so the wave's events lie in
-// c0 .. c0 + 63; a lane whose event reaches the window's last cube (possible
-// only with empty cubes past it) searches the whole array instead.
-__device__ __forceinline__ int wave_find_cube(const int32_t* __restrict__ off,
-                                              const int32_t* __restrict__ n_ev, int n_cubes, int e0,
-                                              int e, int& nev) {
-  const int c0 = find_cube(off, n_cubes, e0);
-  const int lane = threadIdx.x & 63;
-  const int lim = min(n_cubes - c0, 64);
-  const int ok = lane < lim ? off[c0 + lane] : 0x7fffffff;
-  const int nk = lane < lim ? n_ev[c0 + lane] : 0;
-  int lo = 0;
-#pragma unroll
-  for (int stp = 32; stp >= 1; stp >>= 1) {
-    const int probe = lo + stp;  // <= 63
-    const int v = __shfl(ok, probe, 64);
-    if (probe < lim && v <= e) lo = probe;
-  }
-  nev = __shfl(nk, lo, 64);
-  int c = c0 + lo;
-  if (lo == 63) {
-    c = find_cube(off, n_cubes, e);
-    nev = n_ev[c];
-  }
-  return c;
-}
-
-// q = c / n, r = c % n for 0 <= c < 2^24 through an FP32 reciprocal and
-// one correction step
-__device__ __forceinline__ void divmod_small(int c, int n, float rn, int& q, int& r) {
-  q = (int)((float)c * rn);
-  r = c - q * n;
-  if (r < 0) {
-    --q;
-    r += n;
-  } else if (r >= n) {
-    ++q;
-    r -= n;
-  }
-}
-
-// hypercube coordinates, last dimension fastest (vflowplus.py:126-128)
-__device__ __forceinline__ void cube_digits(int cube, int d, int n_strat, float rn,
-                                            int (&h)[MV_FUSED_MAX_DIM]) {
-  int c = cube;
-#pragma unroll
-  for (int j = MV_FUSED_MAX_DIM - 1; j >= 0; --j) {
-    h[j] = 0;
-    if (j < d) {
-      int q, r;
-      divmod_small(c, n_strat, rn, q, r);
-      h[j] = r;
-      c = q;
-    }
-  }
-}
-
-// Segmented suffix sum over the wave's (sorted) cube ids; the first lane of
-// each segment adds the segment total to ress / ress2 (vflowplus.py:213-214).
-__device__ __forceinline__ void segment_add(int cube, double v, double v2, double* __restrict__ ress,
-                                            double* __restrict__ ress2) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double ov = __shfl_down(v, off, 64);
-    const double ov2 = __shfl_down(v2, off, 64);
-    const int oc = __shfl_down(cube, off, 64);
-    if (lane + off < 64 && oc == cube) {
-      v += ov;
-      v2 += ov2;
-    }
-  }
-  const int pc = __shfl_up(cube, 1, 64);
-  const bool head = lane == 0 || pc != cube;
-  if (head && cube >= 0) {
-    unsafeAtomicAdd(&ress[cube], v);
-    unsafeAtomicAdd(&ress2[cube], v2);
-  }
-}
-
-
-
-// ---------------------------------------------------------------------------
-// fused VEGAS+ (vflowplus.py:37-80 + 187-220, monte_carlo.py:249-275)
-// FP64: xn = ((h + u) * 50) / n_strat with three roundings (vflowplus.py:72).
-// Mixed: the same index arithmetic in FP64 from u = hi * 2^-32, FP32 map and
-// integrand, FP64 sums.
-// ---------------------------------------------------------------------------
-template <int FAM, bool TRAIN, bool MIXED>
-__global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_integrand fn,
-                                                      const double* __restrict__ grid,
-                                                      double* __restrict__ ress,
-                                                      double* __restrict__ ress2, RowsK R) {
-  const uint32_t draw = draw_of(s);
-  using T = typename std::conditional<MIXED, float, double>::type;
-  using T2 = typename std::conditional<MIXED, float2, double2>::type;
-  extern __shared__ double smem[];
-  const int d = s.n_dim;
-  const int nb = d * kBins;
-  T2* tab = reinterpret_cast<T2*>(smem);
-  T* tw = reinterpret_cast<T*>(smem + (MIXED ? nb : 2 * nb));
-  double* hist = smem + (MIXED ? nb + (nb + 1) / 2 : 3 * nb);
-  const bool limits = s.xmin != nullptr;
-  for (int i = threadIdx.x; i < nb; i += kBlock) {
-    const int j = i / kBins, b = i - j * kBins;
-    const double lo = grid[j * (kBins + 1) + b];
-    const double dx = grid[j * (kBins + 1) + b + 1] - lo;
-    double P = lo, Q = dx;
-    if (limits) {
-      P = s.xmin[j] + lo * s.xdelta[j];
-      Q = dx * s.xdelta[j];
-    }
-    Integrand<T, FAM>::fold(fn, j, P, Q);
-    T2 e;
-    e.x = (T)P;
-    e.y = (T)Q;
-    tab[i] = e;
-    tw[i] = (T)(dx * kFBins);
-  }
-  if (TRAIN)
-    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double* wh = hist + wave * nb;
-
-  Integrand<T, FAM> F(fn);
-  const double wscale = limits ? s.xjac * s.xdeltajac : s.xjac;
-  const double fstrat = (double)st.n_strat;
-  const float rn = 1.0f / (float)st.n_strat;
-  const int total = st.off[st.n_cubes];
-  double acc = 0.0, acc2 = 0.0;
-  const int64_t step = (int64_t)gridDim.x * kBlock;
-  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
-  for (int64_t base = (int64_t)blockIdx.x * kBlock + wave_u * 64; base < total; base += step) {
-    const int64_t i = base + lane;
-    int nev;
-    const int wcube = wave_find_cube(st.off, st.n_ev, st.n_cubes, (int)base,
-                                     (int)(i < total ? i : total - 1), nev);
-    int cube = -1;
-    double tmp = 0.0, tmp2 = 0.0;
-    if (i < total) {
-      cube = wcube;
-      int h[MV_FUSED_MAX_DIM];
-      cube_digits(cube, d, st.n_strat, rn, h);
-      const uint64_t ge = (uint64_t)(s.event_offset + i);
-      const uint32_t c0 = (uint32_t)ge, c1 = (uint32_t)(ge >> 32);
-      F.begin();
-      T w = (T)1;
-      uint32_t pk[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        if (g < s.groups) {
-          uint4 hw4 = make_uint4(0, 0, 0, 0), lw4 = make_uint4(0, 0, 0, 0);
-          if (!s.inject) {
-            hw4 = philox(c0, c1, draw, (uint32_t)g, s.key);
-            if (!MIXED) lw4 = philox(c0, c1, draw, (uint32_t)(s.groups + g), s.key);
-          }
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int j = 4 * g + q;
-            if (j < d) {
-              double u;
-              if (MIXED) {
-                const uint32_t hw = s.inject ? static_cast<const uint32_t*>(s.inject)[i * d + j]
-                                             : word(hw4, q);
-                u = (double)hw * 0x1p-32;
-              } else {
-                u = s.inject ? static_cast<const double*>(s.inject)[i * d + j]
-                             : draw_f64(word(hw4, q), word(lw4, q));
-              }
-              const double xn = (((double)h[j] + u) * kFBins) / fstrat;  // vflowplus.py:72
-              const int ind = clamp_bin((int)xn);
-              const T frac = (T)(xn - floor(xn));
-              const T2 t = tab[j * kBins + ind];
-              F.add_folded(j, MIXED ? (T)fmaf((float)t.y, (float)frac, (float)t.x)
-                                    : (T)fma((double)t.y, (double)frac, (double)t.x));
-              w = w * tw[j * kBins + ind];
-              if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
-            }
-          }
-        }
-      }
-      // vflowplus.py:77, monte_carlo.py:270-274, vflowplus.py:209
-      const double wev = (double)w / (double)nev;
-      tmp = (wev * wscale) * (double)F.value();
-      tmp2 = tmp * tmp;
-      acc += tmp;
-      acc2 += tmp2;
-      if (TRAIN) hist_adds(wh, pk, d, tmp2);
-    }
-    segment_add(cube, tmp, tmp2, ress, ress2);
-  }
-  finish_block<TRAIN, false>(acc, acc2, hist, nb, R);
-}
 
 // ---------------------------------------------------------------------------
 // unfused VEGAS+: sample (vflowplus.py:37-80) and digest (209-218)
@@ -543,19 +327,6 @@ int validate_strata(const mv_strata* st, int n_dim, const char* what) {
 
 StrataK make_strata(const mv_strata& s) { return StrataK{s.n_strat, s.n_cubes, s.n_ev, s.offsets}; }
 
-template <int FAM>
-int launch_vplus_fused(const StepK& k, const StrataK& st, const mv_integrand& fn, bool mixed,
-                       bool train, const double* grid, double* ress, double* ress2, const RowsK& R,
-                       hipStream_t s) {
-  const int nb = k.n_dim * kBins;
-  const size_t hist = train ? (size_t)kWaves * nb : 0;
-  const size_t lds = ((mixed ? nb + (nb + 1) / 2 : 3 * nb) + hist) * sizeof(double);
-#define MV_VPF(TR, MX) \
-  launch_producer(vplus_fused<FAM, TR, MX>, k.n_events, lds, s, k, st, fn, grid, ress, ress2, R)
-  if (mixed) return train ? MV_VPF(true, true) : MV_VPF(false, true);
-  return train ? MV_VPF(true, false) : MV_VPF(false, false);
-#undef MV_VPF
-}
 
 }  // namespace
 
@@ -575,15 +346,10 @@ extern "C" int mv_vplus_fused(const mv_step* step, const mv_strata* strata, cons
   const StepK k = make_stepk(*step);
   const StrataK st = make_strata(*strata);
   const bool mixed = step->precision == MV_MIXED, train = step->train != 0;
-  int blocks = 1;
   hipStream_t s = as_stream(stream);
-  switch (fn->family) {
-    case MV_FN_CONSTANT: blocks = launch_vplus_fused<MV_FN_CONSTANT>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
-    case MV_FN_GAUSSIAN: blocks = launch_vplus_fused<MV_FN_GAUSSIAN>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
-    case MV_FN_PRODUCT: blocks = launch_vplus_fused<MV_FN_PRODUCT>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
-    case MV_FN_GAUSSMIX: blocks = launch_vplus_fused<MV_FN_GAUSSMIX>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
-    case MV_FN_BWPOWER: blocks = launch_vplus_fused<MV_FN_BWPOWER>(k, st, *fn, mixed, train, grid, ress, ress2, R, s); break;
-  }
+  const int blocks = mixed ? launch_vplus_fused_mixed(fn->family, train, k, st, *fn, grid, ress, ress2, R, s)
+                           : launch_vplus_fused_f64(fn->family, train, k, st, *fn, grid, ress, ress2, R, s);
+  if (blocks < 0) return blocks;
   if ((rc = check_launch("mv_vplus_fused"))) return rc;
   return rows_used(blocks);
 }

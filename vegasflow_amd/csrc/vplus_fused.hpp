@@ -1,0 +1,236 @@
+// vplus_fused.hpp - the fused VEGAS+ producer (vflowplus.py:37-80 + 187-220,
+// monte_carlo.py:249-275), compiled per (integrand family, train, precision,
+// n_dim, injected draws) for n_dim 1..MV_FUSED_MAX_DIM, so that the event
+// body is straight-line code; vplus_fused_mixed.hip and vplus_fused_f64.hip
+// instantiate one precision each.
+//
+// FP64: xn = ((h + u) * 50) / n_strat with three roundings (vflowplus.py:72).
+// Mixed: the same index arithmetic in FP64 from u = hi * 2^-32, FP32 map and
+// integrand, FP64 sums.
+//
+// Every wave owns one contiguous run of the iteration's events, taken 64 at
+// a time. The run's first cube comes from one binary search over the
+// offsets; each later trip starts in the cube where the previous one ended,
+// so a lane finds its cube by a lifting over the next 64 offsets only. Per
+// cube sums: one wave reduction when the trip lies in a single cube, else a
+// segmented scan; one FP64 global atomic per (trip, cube) segment.
+#pragma once
+
+#include <array>
+#include <type_traits>
+#include <utility>
+
+#include "abi_util.hpp"
+#include "device.hpp"
+#include "fused_event.hpp"
+#include "rows.hpp"
+#include "vplus.hpp"
+
+namespace mv {
+
+namespace {
+
+// per-cube sums of one trip (vflowplus.py:213-214)
+__device__ __forceinline__ void trip_cube_sums(int cube, double v, double v2, double* __restrict__ ress,
+                                               double* __restrict__ ress2) {
+  const int first = __builtin_amdgcn_readlane(cube, 0);
+  const int last = __builtin_amdgcn_readlane(cube, 63);
+  if (first == last) {
+    v = wave_sum(v);
+    v2 = wave_sum(v2);
+    if ((threadIdx.x & 63) == 0) {
+      unsafeAtomicAdd(&ress[first], v);
+      unsafeAtomicAdd(&ress2[first], v2);
+    }
+  } else {
+    segment_add(cube, v, v2, ress, ress2);
+  }
+}
+
+// One event of cube ``cube`` (nev events): returns tmp = w f / nev and, when
+// training, adds tmp^2 to the wave's histogram wh. INJ: the draws come from
+// s.inject (event i) instead of Philox.
+template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, typename T, typename T2>
+__device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK& s, uint32_t draw, int cube,
+                                              int nev, int64_t i, int n_strat, double fstrat, float rn,
+                                              const T2* tab, const T* tw, double wscale, double* wh) {
+  constexpr int groups = (D + 3) / 4;
+  int h[D];
+  cube_digits_d<D>(cube, n_strat, rn, h);
+  const uint64_t ge = (uint64_t)(s.event_offset + i);
+  const uint32_t c0w = (uint32_t)ge, c1w = (uint32_t)(ge >> 32);
+  F.begin();
+  T w = (T)1;
+  uint32_t pk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int g = 0; g < groups; ++g) {
+    uint4 hw4 = make_uint4(0, 0, 0, 0), lw4 = make_uint4(0, 0, 0, 0);
+    if (!INJ) {
+      hw4 = philox(c0w, c1w, draw, (uint32_t)g, s.key);
+      if (!MIXED) lw4 = philox(c0w, c1w, draw, (uint32_t)(groups + g), s.key);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = 4 * g + q;
+      if (j < D) {
+        double u;
+        if (MIXED) {
+          const uint32_t hw = INJ ? static_cast<const uint32_t*>(s.inject)[i * D + j] : word(hw4, q);
+          u = (double)hw * 0x1p-32;
+        } else {
+          u = INJ ? static_cast<const double*>(s.inject)[i * D + j] : draw_f64(word(hw4, q), word(lw4, q));
+        }
+        const double xn = (((double)h[j] + u) * kFBins) / fstrat;  // vflowplus.py:72
+        const int ind = clamp_bin((int)xn);
+        const T frac = (T)(xn - floor(xn));
+        const T2 t = tab[j * kBins + ind];
+        F.add_folded(j, MIXED ? (T)fmaf((float)t.y, (float)frac, (float)t.x)
+                              : (T)fma((double)t.y, (double)frac, (double)t.x));
+        w = w * tw[j * kBins + ind];
+        if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
+      }
+    }
+  }
+  // vflowplus.py:77, monte_carlo.py:270-274, vflowplus.py:209
+  const double wev = (double)w / (double)nev;
+  const double tmp = (wev * wscale) * (double)F.value();
+  if (TRAIN) hist_adds(wh, pk, D, tmp * tmp);
+  return tmp;
+}
+
+// The wave's run of events first .. end - 1, 64 per trip, into acc, acc2
+// (per-lane sums) and the per-cube sums ress, ress2.
+template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, typename T, typename T2>
+__device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, uint32_t draw,
+                                          const StrataK& st, int64_t first, int64_t end, double fstrat,
+                                          float rn, const T2* tab, const T* tw, double wscale, double* wh,
+                                          double* __restrict__ ress, double* __restrict__ ress2,
+                                          double& acc, double& acc2) {
+  const int lane = threadIdx.x & 63;
+  int c0 = first < end ? find_cube(st.off, st.n_cubes, (int)first) : 0;
+  for (int64_t base = first; base < end; base += 64) {
+    const int64_t i = base + lane;
+    const bool live = i < end;
+    int nev;
+    const int cube = window_cube(st.off, st.n_ev, st.n_cubes, c0, (int)(live ? i : end - 1), nev);
+    double tmp = 0.0, tmp2 = 0.0;
+    if (live) {
+      tmp = vplus_event<FAM, TRAIN, MIXED, D, INJ>(F, s, draw, cube, nev, i, st.n_strat, fstrat, rn, tab,
+                                                   tw, wscale, wh);
+      tmp2 = tmp * tmp;
+      acc += tmp;
+      acc2 += tmp2;
+    }
+    // the next trip starts in the cube of this trip's last event
+    c0 = __builtin_amdgcn_readlane(cube, 63);
+    trip_cube_sums(cube, tmp, tmp2, ress, ress2);
+  }
+}
+
+}  // namespace
+
+template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ>
+__global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_integrand fn,
+                                                      const double* __restrict__ grid,
+                                                      double* __restrict__ ress,
+                                                      double* __restrict__ ress2, RowsK R) {
+  const uint32_t draw = draw_of(s);
+  using T = typename std::conditional<MIXED, float, double>::type;
+  using T2 = typename std::conditional<MIXED, float2, double2>::type;
+  extern __shared__ double smem[];
+  constexpr int nb = D * kBins;
+  T2* tab = reinterpret_cast<T2*>(smem);
+  T* tw = reinterpret_cast<T*>(smem + (MIXED ? nb : 2 * nb));
+  double* hist = smem + (MIXED ? nb + (nb + 1) / 2 : 3 * nb);
+  const bool limits = s.xmin != nullptr;
+  for (int i = threadIdx.x; i < nb; i += kBlock) {
+    const int j = i / kBins, b = i - j * kBins;
+    const double lo = grid[j * (kBins + 1) + b];
+    const double dx = grid[j * (kBins + 1) + b + 1] - lo;
+    double P = lo, Q = dx;
+    if (limits) {
+      P = s.xmin[j] + lo * s.xdelta[j];
+      Q = dx * s.xdelta[j];
+    }
+    Integrand<T, FAM>::fold(fn, j, P, Q);
+    T2 e;
+    e.x = (T)P;
+    e.y = (T)Q;
+    tab[i] = e;
+    tw[i] = (T)(dx * kFBins);
+  }
+  if (TRAIN)
+    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double* wh = hist + wave * nb;
+
+  Integrand<T, FAM> F(fn);
+  const double wscale = limits ? s.xjac * s.xdeltajac : s.xjac;
+  const double fstrat = (double)st.n_strat;
+  const float rn = 1.0f / (float)st.n_strat;
+  const int64_t total = st.off[st.n_cubes];
+  // this wave's run: trips gw * trips / nwaves up to (gw + 1) * trips / nwaves
+  // (exclusive) of 64 events each, the trips split evenly over the waves
+  const int64_t trips = (total + 63) >> 6;
+  const int64_t nwaves = (int64_t)gridDim.x * kWaves;
+  const int64_t gw = (int64_t)blockIdx.x * kWaves + wave;
+  const int64_t first = (gw * trips / nwaves) * 64;
+  const int64_t end = min(((gw + 1) * trips / nwaves) * 64, total);
+  double acc = 0.0, acc2 = 0.0;
+  vplus_run<FAM, TRAIN, MIXED, D, INJ>(F, s, draw, st, first, end, fstrat, rn, tab, tw, wscale, wh, ress,
+                                       ress2, acc, acc2);
+  finish_block<TRAIN, false>(acc, acc2, hist, nb, R);
+}
+
+using VplusKernel = void (*)(StepK, StrataK, mv_integrand, const double*, double*, double*, RowsK);
+
+template <int FAM, bool TRAIN, bool MIXED, bool INJ, int... Ds>
+constexpr std::array<VplusKernel, sizeof...(Ds)> vplus_kernels_of(std::integer_sequence<int, Ds...>) {
+  return {&vplus_fused<FAM, TRAIN, MIXED, Ds + 1, INJ>...};
+}
+
+template <int FAM, bool TRAIN, bool MIXED, bool INJ>
+int launch_vplus_fam(const StepK& k, const StrataK& st, const mv_integrand& fn, const double* grid,
+                     double* ress, double* ress2, const RowsK& R, hipStream_t s) {
+  static constexpr auto table =
+      vplus_kernels_of<FAM, TRAIN, MIXED, INJ>(std::make_integer_sequence<int, MV_FUSED_MAX_DIM>{});
+  if (k.n_dim < 1 || k.n_dim > MV_FUSED_MAX_DIM)
+    return fail(MV_ERR_UNSUPPORTED, "fused VEGAS+ kernel: n_dim %d outside 1..%d", k.n_dim,
+                MV_FUSED_MAX_DIM);
+  const int nb = k.n_dim * kBins;
+  const size_t hist = TRAIN ? (size_t)kWaves * nb : 0;
+  const size_t lds = ((MIXED ? nb + (nb + 1) / 2 : 3 * nb) + hist) * sizeof(double);
+  return launch_producer(table[k.n_dim - 1], k.n_events, lds, s, k, st, fn, grid, ress, ress2, R);
+}
+
+template <int FAM, bool MIXED>
+int launch_vplus_family(bool train, const StepK& k, const StrataK& st, const mv_integrand& fn,
+                        const double* grid, double* ress, double* ress2, const RowsK& R, hipStream_t s) {
+  const bool inj = k.inject != nullptr;
+  if (train)
+    return inj ? launch_vplus_fam<FAM, true, MIXED, true>(k, st, fn, grid, ress, ress2, R, s)
+               : launch_vplus_fam<FAM, true, MIXED, false>(k, st, fn, grid, ress, ress2, R, s);
+  return inj ? launch_vplus_fam<FAM, false, MIXED, true>(k, st, fn, grid, ress, ress2, R, s)
+             : launch_vplus_fam<FAM, false, MIXED, false>(k, st, fn, grid, ress, ress2, R, s);
+}
+
+// The launcher of one precision: the kernel for (family, train, n_dim,
+// injected draws). Returns the workgroups launched or a negative error.
+template <bool MIXED>
+int launch_vplus_precision(int family, bool train, const StepK& k, const StrataK& st,
+                           const mv_integrand& fn, const double* grid, double* ress, double* ress2,
+                           const RowsK& R, hipStream_t s) {
+#define MV_VPF(F) return launch_vplus_family<F, MIXED>(train, k, st, fn, grid, ress, ress2, R, s)
+  switch (family) {
+    case MV_FN_CONSTANT: MV_VPF(MV_FN_CONSTANT);
+    case MV_FN_GAUSSIAN: MV_VPF(MV_FN_GAUSSIAN);
+    case MV_FN_PRODUCT: MV_VPF(MV_FN_PRODUCT);
+    case MV_FN_GAUSSMIX: MV_VPF(MV_FN_GAUSSMIX);
+    case MV_FN_BWPOWER: MV_VPF(MV_FN_BWPOWER);
+    default: return fail(MV_ERR_UNSUPPORTED, "fused VEGAS+ kernel: unknown family %d", family);
+  }
+#undef MV_VPF
+}
+
+}  // namespace mv
