@@ -179,21 +179,24 @@ __device__ __forceinline__ void finish_row(double acc, double acc2, const double
 }
 
 // Workgroup end of the pair-histogram producers (BS threads): sums as in
-// finish_row, histogram bins from the pair tables (pair_marginal).
-template <int BS>
+// finish_row, histogram bins from the pair tables (pair_marginal). SUMS =
+// false leaves columns 0 and 1 alone (VEGAS+ sums per hypercube).
+template <int BS, bool SUMS = true>
 __device__ __forceinline__ void finish_pairs(double acc, double acc2, const double* H, int d,
                                              double* row) {
   constexpr int W = BS / 64;
   __shared__ double red[2 * W];
-  acc = wave_sum(acc);
-  acc2 = wave_sum(acc2);
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[wave] = acc;
-    red[W + wave] = acc2;
+  if (SUMS) {
+    acc = wave_sum(acc);
+    acc2 = wave_sum(acc2);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      red[wave] = acc;
+      red[W + wave] = acc2;
+    }
   }
   __syncthreads();  // also: every histogram add has landed
-  if (threadIdx.x < 2) {
+  if (SUMS && threadIdx.x < 2) {
     const double* r = red + threadIdx.x * W;
     double a = r[0];
     for (int k = 1; k < W; ++k) a += r[k];

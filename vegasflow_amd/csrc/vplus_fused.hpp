@@ -28,6 +28,20 @@
 
 namespace mv {
 
+// Training at kPairMinDim <= n_dim <= kPairMaxDim keeps pair histograms
+// (fused_event.hpp) in kPairBlock-thread workgroups: the events of a wave
+// lie in one or two hypercubes, so per-dimension histograms would take most
+// adds of a wave on the few bins a cube spans; the pair tables spread them
+// over up to 2500 cells and need one add per pair of dimensions.
+template <bool TRAIN, int D>
+constexpr bool vplus_pairs() {
+  return TRAIN && D >= kPairMinDim && D <= kPairMaxDim;
+}
+template <bool TRAIN, int D>
+constexpr int vplus_block() {
+  return vplus_pairs<TRAIN, D>() ? kPairBlock : kBlock;
+}
+
 namespace {
 
 // per-cube sums of one trip (vflowplus.py:213-214)
@@ -48,8 +62,9 @@ __device__ __forceinline__ void trip_cube_sums(int cube, double v, double v2, do
 }
 
 // One event of cube ``cube`` (nev events): returns tmp = w f / nev and, when
-// training, adds tmp^2 to the wave's histogram wh. INJ: the draws come from
-// s.inject (event i) instead of Philox.
+// training, adds tmp^2 to the histograms wh (the workgroup's pair tables, or
+// the wave's per-dimension histogram). INJ: the draws come from s.inject
+// (event i) instead of Philox.
 template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, typename T, typename T2>
 __device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK& s, uint32_t draw, int cube,
                                               int nev, int64_t i, int n_strat, double fstrat, float rn,
@@ -94,7 +109,10 @@ __device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK&
   // vflowplus.py:77, monte_carlo.py:270-274, vflowplus.py:209
   const double wev = (double)w / (double)nev;
   const double tmp = (wev * wscale) * (double)F.value();
-  if (TRAIN) hist_adds(wh, pk, D, tmp * tmp);
+  if (vplus_pairs<TRAIN, D>())
+    pair_adds(wh, pk, D, tmp * tmp);
+  else if (TRAIN)
+    hist_adds(wh, pk, D, tmp * tmp);
   return tmp;
 }
 
@@ -130,10 +148,13 @@ __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, 
 }  // namespace
 
 template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ>
-__global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_integrand fn,
+__global__ __launch_bounds__((vplus_block<TRAIN, D>())) void vplus_fused(StepK s, StrataK st, mv_integrand fn,
                                                       const double* __restrict__ grid,
                                                       double* __restrict__ ress,
                                                       double* __restrict__ ress2, RowsK R) {
+  constexpr bool PAIRS = vplus_pairs<TRAIN, D>();
+  constexpr int BS = vplus_block<TRAIN, D>();
+  constexpr int W = BS / 64;
   const uint32_t draw = draw_of(s);
   using T = typename std::conditional<MIXED, float, double>::type;
   using T2 = typename std::conditional<MIXED, float2, double2>::type;
@@ -143,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_in
   T* tw = reinterpret_cast<T*>(smem + (MIXED ? nb : 2 * nb));
   double* hist = smem + (MIXED ? nb + (nb + 1) / 2 : 3 * nb);
   const bool limits = s.xmin != nullptr;
-  for (int i = threadIdx.x; i < nb; i += kBlock) {
+  for (int i = threadIdx.x; i < nb; i += BS) {
     const int j = i / kBins, b = i - j * kBins;
     const double lo = grid[j * (kBins + 1) + b];
     const double dx = grid[j * (kBins + 1) + b + 1] - lo;
@@ -159,11 +180,11 @@ __global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_in
     tab[i] = e;
     tw[i] = (T)(dx * kFBins);
   }
-  if (TRAIN)
-    for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
+  constexpr int hist_len = PAIRS ? pair_cells(D) : (TRAIN ? kWaves * nb : 0);
+  for (int i = threadIdx.x; i < hist_len; i += BS) hist[i] = 0.0;
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  double* wh = hist + wave * nb;
+  double* wh = PAIRS ? hist : hist + wave * nb;
 
   Integrand<T, FAM> F(fn);
   const double wscale = limits ? s.xjac * s.xdeltajac : s.xjac;
@@ -173,14 +194,17 @@ __global__ __launch_bounds__(kBlock) void vplus_fused(StepK s, StrataK st, mv_in
   // this wave's run: trips gw * trips / nwaves up to (gw + 1) * trips / nwaves
   // (exclusive) of 64 events each, the trips split evenly over the waves
   const int64_t trips = (total + 63) >> 6;
-  const int64_t nwaves = (int64_t)gridDim.x * kWaves;
-  const int64_t gw = (int64_t)blockIdx.x * kWaves + wave;
+  const int64_t nwaves = (int64_t)gridDim.x * W;
+  const int64_t gw = (int64_t)blockIdx.x * W + wave;
   const int64_t first = (gw * trips / nwaves) * 64;
   const int64_t end = min(((gw + 1) * trips / nwaves) * 64, total);
   double acc = 0.0, acc2 = 0.0;
   vplus_run<FAM, TRAIN, MIXED, D, INJ>(F, s, draw, st, first, end, fstrat, rn, tab, tw, wscale, wh, ress,
                                        ress2, acc, acc2);
-  finish_block<TRAIN, false>(acc, acc2, hist, nb, R);
+  if (PAIRS)
+    finish_pairs<BS, false>(acc, acc2, hist, D, group_row(R));
+  else
+    finish_block<TRAIN, false>(acc, acc2, hist, nb, R);
 }
 
 using VplusKernel = void (*)(StepK, StrataK, mv_integrand, const double*, double*, double*, RowsK);
@@ -199,8 +223,11 @@ int launch_vplus_fam(const StepK& k, const StrataK& st, const mv_integrand& fn, 
     return fail(MV_ERR_UNSUPPORTED, "fused VEGAS+ kernel: n_dim %d outside 1..%d", k.n_dim,
                 MV_FUSED_MAX_DIM);
   const int nb = k.n_dim * kBins;
-  const size_t hist = TRAIN ? (size_t)kWaves * nb : 0;
+  const bool pairs = TRAIN && k.n_dim >= kPairMinDim && k.n_dim <= kPairMaxDim;
+  const size_t hist = pairs ? (size_t)pair_cells(k.n_dim) : (TRAIN ? (size_t)kWaves * nb : 0);
   const size_t lds = ((MIXED ? nb + (nb + 1) / 2 : 3 * nb) + hist) * sizeof(double);
+  if (pairs)
+    return launch_producer<kPairBlock>(table[k.n_dim - 1], k.n_events, lds, s, k, st, fn, grid, ress, ress2, R);
   return launch_producer(table[k.n_dim - 1], k.n_events, lds, s, k, st, fn, grid, ress, ress2, R);
 }
 
