@@ -132,6 +132,50 @@ def test_vplus_events_limit_and_adaptive_rules():
     assert inst._events_bound() >= inst._n_cubes * inst.min_neval_hcube + 500_000
 
 
+def test_strata_division_by_fma_corrections_is_correctly_rounded():
+    """The fused VEGAS+ kernel divides t = (h + u) * 50 by n_strat with
+    y = 1/n_strat and two FMA residual corrections (div_by_strat in
+    vplus.hpp). FMA is emulated exactly here (a rational sum rounded once),
+    and the quotient must equal the IEEE division the reference performs
+    (vflowplus.py:72) on draws, the technical cut, and points within a few
+    ulp of bin edges, where a wrong last bit would change the bin."""
+    from fractions import Fraction
+
+    def fma(a, b, c):
+        return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+    def div5(t, n, y):
+        q = t * y
+        r = fma(-q, n, t)
+        q = fma(r, y, q)
+        r = fma(-q, n, t)
+        return fma(r, y, q)
+
+    rng = np.random.default_rng(72)
+    checked = 0
+    for n in list(range(1, 33)) + [47, 64, 99, 100, 127]:
+        fn, y = float(n), 1.0 / n
+        for k in range(120):
+            h = int(rng.integers(n))
+            if k % 4 == 0:
+                u = float(rng.integers(1 << 53)) * 2.0**-53
+            elif k % 4 == 1:
+                u = float(rng.integers(1 << 32)) * 2.0**-32
+            elif k % 4 == 2:
+                u = 1e-8 + (1.0 - 2e-8) * float(rng.random())
+            else:  # t within a few ulp of an edge m * n / 50
+                m = int(rng.integers(50 * n))
+                u = m / 50.0 - h + int(rng.integers(-1000, 1001)) * 2.0**-52
+                if not 0.0 <= u < 1.0:
+                    u = 0.5
+            t = (h + u) * 50.0
+            q = div5(t, fn, y)
+            assert q == t / fn, (n, h, u)
+            assert int(q) == int(t / fn)
+            checked += 1
+    assert checked == 37 * 120
+
+
 def test_grid_json_roundtrip_and_errors():
     """test_algs.py:102-148 on host data."""
     from vegasflow_amd import VegasFlow

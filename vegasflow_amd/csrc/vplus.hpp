@@ -75,6 +75,20 @@ __device__ __forceinline__ void divmod_small(int c, int n, float rn, int& q, int
   }
 }
 
+// t / n rounded correctly, for a positive integer n and 0 <= t < 2^31, from
+// y = 1 / n rounded correctly (one division per launch): q = t y is within
+// 1.5 ulp; one FMA residual correction makes it faithful, and a second,
+// from a faithful quotient and a correctly rounded reciprocal, gives the
+// correctly rounded quotient (Markstein's theorem). Five FP64 operations
+// instead of a full division per dimension.
+__device__ __forceinline__ double div_by_strat(double t, double n, double y) {
+  double q = t * y;
+  double r = fma(-q, n, t);
+  q = fma(r, y, q);
+  r = fma(-q, n, t);
+  return fma(r, y, q);
+}
+
 // hypercube coordinates for a compile-time n_dim D, last dimension fastest
 // (vflowplus.py:126-128)
 template <int D>

@@ -4,7 +4,8 @@
 // body is straight-line code; vplus_fused_mixed.hip and vplus_fused_f64.hip
 // instantiate one precision each.
 //
-// FP64: xn = ((h + u) * 50) / n_strat with three roundings (vflowplus.py:72).
+// FP64: xn = ((h + u) * 50) / n_strat with three roundings (vflowplus.py:72),
+// the division by n_strat as div_by_strat (vplus.hpp), correctly rounded.
 // Mixed: the same index arithmetic in FP64 from u = hi * 2^-32, FP32 map and
 // integrand, FP64 sums.
 //
@@ -67,7 +68,8 @@ __device__ __forceinline__ void trip_cube_sums(int cube, double v, double v2, do
 // (event i) instead of Philox.
 template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, typename T, typename T2>
 __device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK& s, uint32_t draw, int cube,
-                                              int nev, int64_t i, int n_strat, double fstrat, float rn,
+                                              int nev, int64_t i, int n_strat, double fstrat, double ystrat,
+                                              float rn,
                                               const T2* tab, const T* tw, double wscale, double* wh) {
   constexpr int groups = (D + 3) / 4;
   int h[D];
@@ -95,7 +97,7 @@ __device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK&
         } else {
           u = INJ ? static_cast<const double*>(s.inject)[i * D + j] : draw_f64(word(hw4, q), word(lw4, q));
         }
-        const double xn = (((double)h[j] + u) * kFBins) / fstrat;  // vflowplus.py:72
+        const double xn = div_by_strat(((double)h[j] + u) * kFBins, fstrat, ystrat);  // vflowplus.py:72
         const int ind = clamp_bin((int)xn);
         const T frac = (T)(xn - floor(xn));
         const T2 t = tab[j * kBins + ind];
@@ -121,8 +123,9 @@ __device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK&
 template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, typename T, typename T2>
 __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, uint32_t draw,
                                           const StrataK& st, int64_t first, int64_t end, double fstrat,
-                                          float rn, const T2* tab, const T* tw, double wscale, double* wh,
-                                          double* __restrict__ ress, double* __restrict__ ress2,
+                                          double ystrat, float rn, const T2* tab, const T* tw,
+                                          double wscale, double* wh, double* __restrict__ ress,
+                                          double* __restrict__ ress2,
                                           double& acc, double& acc2) {
   const int lane = threadIdx.x & 63;
   int c0 = first < end ? find_cube(st.off, st.n_cubes, (int)first) : 0;
@@ -133,8 +136,8 @@ __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, 
     const int cube = window_cube(st.off, st.n_ev, st.n_cubes, c0, (int)(live ? i : end - 1), nev);
     double tmp = 0.0, tmp2 = 0.0;
     if (live) {
-      tmp = vplus_event<FAM, TRAIN, MIXED, D, INJ>(F, s, draw, cube, nev, i, st.n_strat, fstrat, rn, tab,
-                                                   tw, wscale, wh);
+      tmp = vplus_event<FAM, TRAIN, MIXED, D, INJ>(F, s, draw, cube, nev, i, st.n_strat, fstrat, ystrat, rn,
+                                                   tab, tw, wscale, wh);
       tmp2 = tmp * tmp;
       acc += tmp;
       acc2 += tmp2;
@@ -199,8 +202,8 @@ __global__ __launch_bounds__((vplus_block<TRAIN, D>())) void vplus_fused(StepK s
   const int64_t first = (gw * trips / nwaves) * 64;
   const int64_t end = min(((gw + 1) * trips / nwaves) * 64, total);
   double acc = 0.0, acc2 = 0.0;
-  vplus_run<FAM, TRAIN, MIXED, D, INJ>(F, s, draw, st, first, end, fstrat, rn, tab, tw, wscale, wh, ress,
-                                       ress2, acc, acc2);
+  vplus_run<FAM, TRAIN, MIXED, D, INJ>(F, s, draw, st, first, end, fstrat, 1.0 / fstrat, rn, tab, tw, wscale,
+                                       wh, ress, ress2, acc, acc2);
   if (PAIRS)
     finish_pairs<BS, false>(acc, acc2, hist, D, group_row(R));
   else
