@@ -48,8 +48,11 @@ def mix_np(x):
     return ora.gaussian_mix(x, (0.5, 0.5), (0.05, 0.1), (0.33, 0.67))
 
 
-def _setup(d, n, adaptive, rng, ragged=True):
+def _setup(d, n, adaptive, rng, ragged=True, n_strat=None):
     st = ora.vplus_setup(d, n, adaptive)
+    if n_strat is not None:  # more cubes than the reference's 1e4 cap
+        st["n_strat"], st["n_cubes"] = n_strat, n_strat**d
+        st["min_neval"] = 2
     if ragged:
         st["n_ev"] = rng.integers(st["min_neval"], st["min_neval"] + 40, size=st["n_cubes"]).astype(np.int32)
     return st
@@ -87,10 +90,13 @@ def test_vplus_sample_bitwise(d, n):
 
 
 @pytest.mark.parametrize("adaptive", [True, False])
-@pytest.mark.parametrize("d,n", [(2, 20000), (4, 200000), (5, 100000)])
-def test_vplus_fused_iteration_matches_oracle(d, n, adaptive):
+@pytest.mark.parametrize("d,n,n_strat", [(2, 20000, None), (4, 200000, None), (5, 100000, None),
+                                         (2, 200000, 150)])
+def test_vplus_fused_iteration_matches_oracle(d, n, n_strat, adaptive):
+    """n_strat = 150 at d = 2 gives 22500 cubes, past the reference's cap of
+    1e4 (the engine takes up to 2^24 - 1)."""
     rng = np.random.default_rng(10 * d + adaptive)
-    st = _setup(d, n, adaptive, rng)
+    st = _setup(d, n, adaptive, rng, n_strat=n_strat)
     grid = random_grid(rng, d)
     n_ev, offsets, strata = _device_strata(st)
     total = int(st["n_ev"].sum())
