@@ -246,18 +246,22 @@ class VegasFlowPlus(VegasFlow):
         self._last_out = out
         return out[0], out[1]
 
-    def run_integration(self, n_iter, log_time=True, histograms=None):
-        result = super().run_integration(n_iter, log_time=log_time, histograms=histograms)
-        pend = self._pending_events
-        dev = [e.reshape(1) for e in pend if torch.is_tensor(e)]
-        vals = iter(torch.cat(dev).cpu().tolist() if dev else [])  # one copy for the run
-        for e in pend:
+    def _device_extras(self, replayed):
+        # the event counts still on the device, then the next iteration's
+        last = self._replay_last if replayed else self._last_out
+        pend = [e.reshape(()) for e in self._pending_events if torch.is_tensor(e)]
+        self._extras_layout = (len(pend), last is not None)
+        return pend + ([last[2].reshape(())] if last is not None else [])
+
+    def _take_extras(self, values):
+        n_pend, has_last = self._extras_layout
+        vals = iter(values[:n_pend].tolist())
+        for e in self._pending_events:
             self.events_history.append(int(next(vals)) if torch.is_tensor(e) else int(e))
         self._pending_events = []
-        if self._last_out is not None:
-            self._n_events = int(self._last_out[2].item())
-            self._last_out = None
-        return result
+        if has_last:
+            self._n_events = int(values[n_pend])
+        self._last_out = None
 
 
 def vegasflowplus_wrapper(integrand, n_dim, n_iter, total_n_events, **kwargs):
