@@ -91,10 +91,11 @@ def test_vplus_sample_bitwise(d, n):
 
 @pytest.mark.parametrize("adaptive", [True, False])
 @pytest.mark.parametrize("d,n,n_strat", [(2, 20000, None), (4, 200000, None), (5, 100000, None),
-                                         (2, 200000, 150)])
+                                         (2, 200000, 150), (1, 30000, None), (15, 40000, None)])
 def test_vplus_fused_iteration_matches_oracle(d, n, n_strat, adaptive):
     """n_strat = 150 at d = 2 gives 22500 cubes, past the reference's cap of
-    1e4 (the engine takes up to 2^24 - 1)."""
+    1e4 (the engine takes up to 2^24 - 1). d = 2..14 train through pair
+    tables, d = 1 and 15 through per-dimension histograms."""
     rng = np.random.default_rng(10 * d + adaptive)
     st = _setup(d, n, adaptive, rng, n_strat=n_strat)
     grid = random_grid(rng, d)
@@ -119,11 +120,23 @@ def test_vplus_fused_iteration_matches_oracle(d, n, n_strat, adaptive):
     assert rel(o[1], sig) < 1e-10
     assert rel(hist.cpu().numpy(), o_arr) < 1e-12
     np.testing.assert_allclose(g.cpu().numpy(), new_div, rtol=1e-12, atol=1e-15)
+    # arr_var = ress2 * n_ev - ress^2 cancels where f is nearly flat over a
+    # cube (narrow cubes, e.g. 7500 of them at d = 1): the summation-order
+    # rounding of ress (a few ulp) then scales with ress^2, not with arr_var
+    x_o, w_o, ind_o, seg_o = ora.generate_samples_in_hypercubes(
+        u, st["n_strat"], st["n_ev"], ora.hypercubes(st["n_strat"], d), grid)
+    x_o, w_o = ora.apply_xjac_and_limits(x_o, w_o, 1.0 / st["n_cubes"])
+    o_ress, _, _ = ora.vplus_event(x_o, w_o, ind_o, seg_o, st["n_ev"], mix_np(x_o), False)
+    cancel = np.square(o_ress)
     scale = np.abs(o_var).max()
-    np.testing.assert_allclose(arr_var.cpu().numpy(), o_var, rtol=1e-9, atol=1e-12 * scale)
+    got_var = arr_var.cpu().numpy()
+    assert np.all(np.abs(got_var - o_var) <= 1e-9 * np.abs(o_var) + 1e-12 * scale + 1e-13 * cancel)
     got_n_ev = n_ev.cpu().numpy()
     diff = np.abs(got_n_ev.astype(np.int64) - new_n_ev.astype(np.int64))
-    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    assert diff.max() <= 1
+    well = np.abs(o_var) > 1e-6 * cancel  # variances not dominated by that rounding
+    if well.any():
+        assert (diff[well] > 0).mean() < 1e-3
     off = offsets.cpu().numpy()
     np.testing.assert_array_equal(off[1:], np.cumsum(got_n_ev))
     assert int(o[2]) == int(got_n_ev.sum())
@@ -266,3 +279,93 @@ def test_vplus_finalize_matches_closed_form(n_cubes):
     np.testing.assert_array_equal(o[1:], np.cumsum(got))
     assert int(out[2]) == int(got.sum())
     assert float(ress.abs().max()) == 0.0 and float(ress2.abs().max()) == 0.0
+
+
+def test_vplus_fused_counter_crosses_2_32():
+    """Events whose Philox counter crosses a multiple of 2^32 inside one
+    launch (event_offset near 2^32) draw the oracle's uniforms."""
+    d, n = 3, 30000
+    off = (1 << 32) - 5001
+    rng = np.random.default_rng(44)
+    st = _setup(d, n, False, rng)
+    grid = random_grid(rng, d)
+    n_ev, offsets, strata = _device_strata(st)
+    total = int(st["n_ev"].sum())
+    ress = torch.zeros(st["n_cubes"], dtype=torch.float64, device=DEV)
+    ress2 = torch.zeros_like(ress)
+    ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, True))
+    g = dev(grid)
+    step = kernels.make_step(d, total, precision="f64", train=True, event_offset=off, seed=6, draw_id=1,
+                             xjac=1.0 / st["n_cubes"])
+    ng = kernels.vplus_fused(step, strata, MIX.to_struct(d), g, ress, ress2, ws)
+    out = kernels.vplus_finalize(d, strata, n_ev, offsets, ress, ress2, False, st["min_neval"],
+                                 st["init_calls"], ora.BETA, ws, ng, True, True, g)
+    u = ora.uniform_f64(6, 1, total, d, event_offset=off)
+    res, sig, new_div, _, _, _ = ora.vplus_iteration(grid, st, u, mix_np)
+    o = out.cpu().numpy()
+    assert rel(o[0], res) < 1e-12
+    assert rel(o[1], sig) < 1e-10
+    np.testing.assert_allclose(g.cpu().numpy(), new_div, rtol=1e-12, atol=1e-15)
+
+
+def _mixed_words_u(seed, draw, total, d):
+    # the mixed kernels sample u = (Philox high word) * 2^-32
+    return ora.hi_words(seed, draw, total, d).astype(np.float64) * 2.0**-32
+
+
+@pytest.mark.parametrize("d", [1, 4, 15])
+def test_vplus_mixed_bins_match_oracle(d):
+    """Mixed VEGAS+: the bins are those the oracle computes from the same
+    words (u = hi * 2^-32, FP64 index arithmetic). A constant integrand on the
+    initial grid gives every event the same tmp^2, so the histogram divided by
+    it counts events per bin exactly. d = 4 trains through pair tables, d = 1
+    and 15 through per-dimension histograms."""
+    n = 60000
+    st = ora.vplus_setup(d, n, False)
+    grid = ora.initial_grid(d)
+    n_ev, offsets, strata = _device_strata(st)
+    total = int(st["n_ev"].sum())
+    ress = torch.zeros(st["n_cubes"], dtype=torch.float64, device=DEV)
+    ress2 = torch.zeros_like(ress)
+    ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, True))
+    step = kernels.make_step(d, total, precision="mixed", train=True, seed=12, draw_id=2,
+                             xjac=1.0 / st["n_cubes"])
+    ng = kernels.vplus_fused(step, strata, vfi.Constant(2.5).to_struct(d), dev(grid), ress, ress2, ws)
+    hist = torch.empty((d, 50), dtype=torch.float64, device=DEV)
+    kernels.vplus_finalize(d, strata, n_ev, offsets, ress, ress2, False, st["min_neval"], st["init_calls"],
+                           ora.BETA, ws, ng, True, False, dev(grid), None, hist)
+    u = _mixed_words_u(12, 2, total, d)
+    _, _, ind, _ = ora.generate_samples_in_hypercubes(u, st["n_strat"], st["n_ev"],
+                                                      ora.hypercubes(st["n_strat"], d), grid)
+    tmp2 = (2.5 / st["n_cubes"] / st["min_neval"]) ** 2
+    h = hist.cpu().numpy()
+    for j in range(d):
+        counts = np.bincount(ind[:, j].astype(np.int64), minlength=50)
+        np.testing.assert_allclose(h[j] / tmp2, counts, rtol=1e-6)
+
+
+@pytest.mark.parametrize("d", [4, 15])
+def test_vplus_mixed_matches_oracle_on_words(d):
+    """Mixed VEGAS+ on a random grid and ragged cubes: the FP64 oracle on the
+    same words agrees to FP32 accuracy (map and integrand in FP32)."""
+    n = 200000
+    rng = np.random.default_rng(70 + d)
+    st = _setup(d, n, False, rng)
+    grid = random_grid(rng, d)
+    n_ev, offsets, strata = _device_strata(st)
+    total = int(st["n_ev"].sum())
+    ress = torch.zeros(st["n_cubes"], dtype=torch.float64, device=DEV)
+    ress2 = torch.zeros_like(ress)
+    ws = kernels.Workspace(DEV, kernels.partial_stride(d, 1, True))
+    step = kernels.make_step(d, total, precision="mixed", train=True, seed=31, draw_id=4,
+                             xjac=1.0 / st["n_cubes"])
+    ng = kernels.vplus_fused(step, strata, MIX.to_struct(d), dev(grid), ress, ress2, ws)
+    hist = torch.empty((d, 50), dtype=torch.float64, device=DEV)
+    out = kernels.vplus_finalize(d, strata, n_ev, offsets, ress, ress2, False, st["min_neval"],
+                                 st["init_calls"], ora.BETA, ws, ng, True, False, dev(grid), None, hist)
+    u = _mixed_words_u(31, 4, total, d)
+    res, sig, _, _, _, o_arr = ora.vplus_iteration(grid, st, u, mix_np)
+    o = out.cpu().numpy()
+    assert rel(o[0], res) < 1e-5
+    assert rel(o[1], sig) < 1e-4
+    np.testing.assert_allclose(hist.cpu().numpy(), o_arr, rtol=1e-4, atol=1e-6 * o_arr.max())
