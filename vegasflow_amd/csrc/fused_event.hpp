@@ -115,11 +115,14 @@ __device__ __forceinline__ void event_hist(double* wh, const uint32_t (&pk)[4], 
 // One event, FP64 (vflow.py:87-126 + 33-83 + 389-430). Returns tmp = w f
 // and, when training, adds tmp^2 to the histogram ``wh`` (the wave's
 // per-dimension histograms, or the workgroup's pair histograms if PAIRS).
-template <int FAM, bool TRAIN, bool PAIRS = false>
+// INJ: injected draws taken when inj != nullptr (-1, tested per event), never
+// (0) or always (1); a compile-time 0 or 1 leaves no test in the event body.
+template <int FAM, bool TRAIN, bool PAIRS = false, int INJ = -1>
 __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t c0, uint32_t c1,
                                             uint32_t draw, Key key, int groups, int d,
                                             const double2* tab, const double* tw, double* wh,
                                             const double* inj, double wscale) {
+  const bool injected = INJ < 0 ? inj != nullptr : INJ > 0;
   F.begin();
   double w = 1.0;
   uint32_t pk[4] = {0u, 0u, 0u, 0u};
@@ -127,7 +130,7 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
   for (int g = 0; g < 4; ++g) {
     if (g < groups) {
       uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
-      if (!inj) {
+      if (!injected) {
         h = philox(c0, c1, draw, (uint32_t)g, key);
         l = philox(c0, c1, draw, (uint32_t)(groups + g), key);
       }
@@ -135,7 +138,7 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
       for (int q = 0; q < 4; ++q) {
         const int j = 4 * g + q;
         if (j < d) {
-          const double u = inj ? inj[j] : draw_f64(word(h, q), word(l, q));
+          const double u = injected ? inj[j] : draw_f64(word(h, q), word(l, q));
           const double xn = kFBins * (1.0 - u);  // vflow.py:117
           const int ind = clamp_bin((int)xn);    // vflow.py:67
           const double2 t = tab[j * kBins + ind];
@@ -154,11 +157,12 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
 
 // One event, mixed precision: FP32 map and integrand from the high word of
 // each draw (mixed_position), FP64 result.
-template <int FAM, bool TRAIN, bool PAIRS = false>
+template <int FAM, bool TRAIN, bool PAIRS = false, int INJ = -1>
 __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
                                               uint32_t draw, Key key, int groups, int d,
                                               const float2* tab, float cprod, double* wh,
                                               const uint32_t* inj, double wscale) {
+  const bool injected = INJ < 0 ? inj != nullptr : INJ > 0;
   F.begin();
   float w = 1.0f;
   uint32_t pk[4] = {0u, 0u, 0u, 0u};
@@ -166,13 +170,13 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
   for (int g = 0; g < 4; ++g) {
     if (g < groups) {
       uint4 h = make_uint4(0, 0, 0, 0);
-      if (!inj) h = philox(c0, c1, draw, (uint32_t)g, key);
+      if (!injected) h = philox(c0, c1, draw, (uint32_t)g, key);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int j = 4 * g + q;
         if (j < d) {
           float f1;
-          const int ind = mixed_position(inj ? inj[j] : word(h, q), f1);
+          const int ind = mixed_position(injected ? inj[j] : word(h, q), f1);
           const float2 t = tab[j * kBins + ind];
           F.add_folded(j, fmaf(t.y, f1, t.x));
           w = w * t.y;

@@ -18,7 +18,7 @@
 
 namespace mv {
 
-template <int FAM, bool TRAIN, int D>
+template <int FAM, bool TRAIN, int D, bool INJ>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void vegas_fused_mixed(
     StepK s, mv_integrand fn, const double* __restrict__ grid, RowsK R) {
   const uint32_t draw = draw_of(s);
@@ -38,9 +38,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
-    const double tmp = event_mixed<FAM, TRAIN>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key,
-                                               (D + 3) / 4, D, tab, cprod, wh,
-                                               inj ? inj + i * D : nullptr, wscale);
+    const double tmp = event_mixed<FAM, TRAIN, false, INJ ? 1 : 0>(
+        F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key, (D + 3) / 4, D, tab, cprod, wh,
+        INJ ? inj + i * D : nullptr, wscale);
     acc += tmp;
     acc2 += tmp * tmp;
   }
@@ -51,22 +51,26 @@ namespace {
 
 using Kernel = void (*)(StepK, mv_integrand, const double*, RowsK);
 
-template <int FAM, bool TRAIN, int... Ds>
+template <int FAM, bool TRAIN, bool INJ, int... Ds>
 constexpr std::array<Kernel, sizeof...(Ds)> kernels_of(std::integer_sequence<int, Ds...>) {
-  return {&vegas_fused_mixed<FAM, TRAIN, Ds + 1>...};
+  return {&vegas_fused_mixed<FAM, TRAIN, Ds + 1, INJ>...};
 }
 
 template <int FAM, bool TRAIN>
 int launch_fam(const StepK& k, const mv_integrand& fn, const double* grid, const RowsK& R,
                hipStream_t st) {
+  // draws from Philox, or injected (s.inject, parity tests)
   static constexpr auto table =
-      kernels_of<FAM, TRAIN>(std::make_integer_sequence<int, MV_FUSED_MAX_DIM>{});
+      kernels_of<FAM, TRAIN, false>(std::make_integer_sequence<int, MV_FUSED_MAX_DIM>{});
+  static constexpr auto table_inj =
+      kernels_of<FAM, TRAIN, true>(std::make_integer_sequence<int, MV_FUSED_MAX_DIM>{});
   if (k.n_dim < 1 || k.n_dim > MV_FUSED_MAX_DIM)
     return fail(MV_ERR_UNSUPPORTED, "fused mixed kernel: n_dim %d outside 1..%d", k.n_dim,
                 MV_FUSED_MAX_DIM);
   const int nb = k.n_dim * kBins;
   const size_t lds = (size_t)(nb + (TRAIN ? kWaves * nb : 0)) * sizeof(double);
-  return launch_producer(table[k.n_dim - 1], k.n_events, lds, st, k, fn, grid, R);
+  const Kernel kern = (k.inject ? table_inj : table)[k.n_dim - 1];
+  return launch_producer(kern, k.n_events, lds, st, k, fn, grid, R);
 }
 
 }  // namespace
