@@ -435,7 +435,7 @@ def main():
             "flop_per_event": flops_per_event(D, True),
             "valu_issue": valu_issue("mixed"),
             "f64_kernel": {
-                "kernel": "vegas_fused_f64<GAUSSIAN, train, 8>",
+                "kernel": "vegas_fused_pairs_f64<GAUSSIAN, 8> (FP64 training, n_dim 8)",
                 "achieved": flop / (results["f64"]["fused_kernel_ms"] * 1e-3) / 1e12,
                 "peak": PEAK["f64"],
                 "traffic": TRAFFIC["f64"],

@@ -1,8 +1,9 @@
 // plain_fused.hip - the fused PlainFlow producer (plain.py:18-35,
 // monte_carlo.py:249-275): uniform points from Philox, the built-in
-// integrand, w f sums. Compiled per (integrand family, precision, n_dim) for
-// n_dim 1..MV_FUSED_MAX_DIM, where the event body is straight-line code; a
-// runtime-n_dim instance takes larger n_dim (families that allow it).
+// integrand, w f sums. Compiled per (integrand family, precision, n_dim,
+// injected draws) for n_dim 1..MV_FUSED_MAX_DIM, where the event body is
+// straight-line code; a runtime-n_dim instance takes larger n_dim (families
+// that allow it).
 #include <array>
 #include <type_traits>
 #include <utility>
@@ -13,7 +14,9 @@
 
 namespace mv {
 
-template <int FAM, bool MIXED, int D>
+// INJ: injected draws (s.inject, parity tests) instead of Philox, a
+// compile-time choice so that the event body has no test per dimension.
+template <int FAM, bool MIXED, int D, bool INJ>
 __global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, RowsK R) {
   const uint32_t draw = draw_of(s);
   using T = typename std::conditional<MIXED, float, double>::type;
@@ -29,7 +32,7 @@ __global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, 
     F.begin();
     for (int g = 0; g < groups; ++g) {
       uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
-      if (!s.inject) {
+      if (!INJ) {
         h = philox(c0, c1, draw, (uint32_t)g, s.key);
         if (!MIXED) l = philox(c0, c1, draw, (uint32_t)(groups + g), s.key);
       }
@@ -39,14 +42,13 @@ __global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, 
         if (j < d) {
           T x;
           if (MIXED) {
-            const uint32_t hw = s.inject ? static_cast<const uint32_t*>(s.inject)[i * d + j]
-                                         : word(h, q);
+            const uint32_t hw = INJ ? static_cast<const uint32_t*>(s.inject)[i * d + j] : word(h, q);
             float xf = fmaf((float)(hw >> 9), 0x1p-23f, 0x1p-24f);
             if (limits) xf = fmaf(xf, (float)s.xdelta[j], (float)s.xmin[j]);
             x = (T)xf;
           } else {
-            const double u = s.inject ? static_cast<const double*>(s.inject)[i * d + j]
-                                      : draw_f64(word(h, q), word(l, q));
+            const double u = INJ ? static_cast<const double*>(s.inject)[i * d + j]
+                                 : draw_f64(word(h, q), word(l, q));
             x = (T)(limits ? s.xmin[j] + u * s.xdelta[j] : u);
           }
           F.add(j, x);
@@ -65,17 +67,21 @@ namespace {
 
 using Kernel = void (*)(StepK, mv_integrand, RowsK);
 
-template <int FAM, bool MIXED, int... Ds>
+template <int FAM, bool MIXED, bool INJ, int... Ds>
 constexpr std::array<Kernel, sizeof...(Ds)> kernels_of(std::integer_sequence<int, Ds...>) {
-  return {&plain_fused<FAM, MIXED, Ds + 1>...};
+  return {&plain_fused<FAM, MIXED, Ds + 1, INJ>...};
 }
 
 template <int FAM, bool MIXED>
 int launch_fam(const StepK& k, const mv_integrand& fn, const RowsK& R, hipStream_t st) {
   static constexpr auto table =
-      kernels_of<FAM, MIXED>(std::make_integer_sequence<int, MV_FUSED_MAX_DIM>{});
-  const Kernel kern = k.n_dim >= 1 && k.n_dim <= MV_FUSED_MAX_DIM ? table[k.n_dim - 1]
-                                                                  : &plain_fused<FAM, MIXED, 0>;
+      kernels_of<FAM, MIXED, false>(std::make_integer_sequence<int, MV_FUSED_MAX_DIM>{});
+  static constexpr auto table_inj =
+      kernels_of<FAM, MIXED, true>(std::make_integer_sequence<int, MV_FUSED_MAX_DIM>{});
+  const bool inj = k.inject != nullptr;
+  const Kernel kern = k.n_dim >= 1 && k.n_dim <= MV_FUSED_MAX_DIM
+                          ? (inj ? table_inj : table)[k.n_dim - 1]
+                          : (inj ? &plain_fused<FAM, MIXED, 0, true> : &plain_fused<FAM, MIXED, 0, false>);
   return launch_producer(kern, k.n_events, 0, st, k, fn, R);
 }
 

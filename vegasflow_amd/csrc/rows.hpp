@@ -228,6 +228,12 @@ int launch_plain_fused_dims(int family, bool mixed, const StepK& k, const mv_int
 int launch_vegas_fused_pairs(int family, const StepK& k, const mv_integrand& fn, const double* grid,
                              const RowsK& R, size_t lds, hipStream_t st);
 
+// The FP64 training producer with pair histograms (pairs_f64.hip), as
+// launch_vegas_fused_pairs; ``lds`` = fused_pairs_f64_lds(n_dim).
+size_t fused_pairs_f64_lds(int d);
+int launch_vegas_fused_pairs_f64(int family, const StepK& k, const mv_integrand& fn, const double* grid,
+                                 const RowsK& R, size_t lds, hipStream_t st);
+
 template <bool TRAIN, bool SUMS = true>
 __device__ __forceinline__ void finish_block(double acc, double acc2, const double* hist,
                                              int hist_len, const RowsK& R) {

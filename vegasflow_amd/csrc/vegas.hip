@@ -538,6 +538,9 @@ int launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool 
                        const double* grid, const RowsK& R, hipStream_t st) {
   if (const size_t plds = pairs_lds(k.n_dim, mixed, train))
     return launch_vegas_fused_pairs(FAM, k, fn, grid, R, plds, st);
+  if (!mixed && train && k.n_dim >= kPairMinDim && k.n_dim <= kPairMaxDim &&
+      fused_pairs_f64_lds(k.n_dim) <= max_lds_per_block())
+    return launch_vegas_fused_pairs_f64(FAM, k, fn, grid, R, fused_pairs_f64_lds(k.n_dim), st);
   return mixed ? launch_vegas_fused_mixed(FAM, train, k, fn, grid, R, st)
                : launch_vegas_fused_f64(FAM, train, k, fn, grid, R, st);
 }
