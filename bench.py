@@ -34,12 +34,12 @@ PEAK = {"mixed": 157.3, "f64": 78.6}  # TFLOP/s, MI355X datasheet (FP32 / FP64 v
 # FETCH_SIZE and WRITE_SIZE passes of rocprofv3 scaled by a 1 GiB copy
 # (profiles/vegas_fused_gauss_d8_1e7_pmc.md): 0 bytes per event; the grid
 # reads and the FP64 atomic adds of the partial rows
-TRAFFIC = {"mixed": 0.9748e6, "f64": 5.2603e6}
+TRAFFIC = {"mixed": 0.9748e6, "f64": 0.9830e6}
 TRAFFIC_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_pmc.md"
 # VALU wave instructions per launch (SQ_INSTS_VALU, same profile) and the
 # issue peak they are priced against: one wave64 VALU instruction per SIMD
 # every 4 cycles, 4 SIMDs per CU, 2.4 GHz engine clock (MI355X datasheet)
-VALU_INSTS = {"mixed": 2.535e7, "f64": 5.126e7}
+VALU_INSTS = {"mixed": 2.535e7, "f64": 5.093e7}
 CLOCK_HZ = 2.4e9
 
 
