@@ -305,6 +305,14 @@ int mv_batch_fused_pairs(const mv_batch_item* items, int32_t n_items, const mv_b
                          int32_t n_tiles, int32_t family, int32_t max_dim, const int32_t* draws,
                          const double* grids, double* rows, int32_t blocks, void* stream);
 int32_t mv_batch_pairs_blocks(int32_t family, int32_t max_dim);
+/* The same in MV_F64 (the FP64 event body in the reference's operation
+ * order, pair tables of FP64 cells); mv_batch_pairs_blocks_f64 gives its
+ * resident workgroups. No counterpart in the reference (batching is an
+ * addition). */
+int mv_batch_fused_pairs_f64(const mv_batch_item* items, int32_t n_items, const mv_batch_tile* tiles,
+                             int32_t n_tiles, int32_t family, int32_t max_dim, const int32_t* draws,
+                             const double* grids, double* rows, int32_t blocks, void* stream);
+int32_t mv_batch_pairs_blocks_f64(int32_t family, int32_t max_dim);
 
 /* Iteration end for the items named in ``jobs`` (device int32 pairs
  * {item, dim}: dim = -1 reduces the item's sums into stats (vflow.py:437-438,

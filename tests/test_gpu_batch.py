@@ -60,17 +60,18 @@ def pair_specs():
     ]
 
 
-def test_batch_pair_histograms_match_lone_runs():
-    """Mixed batch through mv_batch_fused_pairs (tiles of 16384 events over
-    1024-thread workgroups), replayed from a graph: every item equals its lone
-    run (odd n_dim, n_dim = 1 and 14 included)."""
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batch_pair_histograms_match_lone_runs(dtype):
+    """Batches through mv_batch_fused_pairs / mv_batch_fused_pairs_f64 (tiles
+    of 16384 events over 1024-thread workgroups), replayed from a graph: every
+    item equals its lone run (odd n_dim, n_dim = 1 and 14 included)."""
     specs = pair_specs()
-    b = BatchVegas(specs, dtype=torch.float32, events_per_tile=16384, pair_histograms=True)
+    b = BatchVegas(specs, dtype=dtype, events_per_tile=16384, pair_histograms=True)
     assert b.pairs and all(v > 0 for v in b._pair_blocks.values())
     b.run(3, graph=True)
     hist = b.history()
     for i, (fn, d, n, seed) in enumerate(specs):
-        lone = vf.VegasFlow(d, n, seed=seed, verbose=False, dtype=torch.float32)
+        lone = vf.VegasFlow(d, n, seed=seed, verbose=False, dtype=dtype)
         lone.compile(fn)
         lone.run_integration(3)
         for k, (r, s, _) in enumerate(lone.history):
@@ -80,12 +81,14 @@ def test_batch_pair_histograms_match_lone_runs():
                                    rtol=1e-10, atol=1e-13)
 
 
-def test_batch_pair_kernel_frozen_items_match_per_wave_kernel():
-    """Items that do not train run the plain mixed event body inside the pair
-    kernel: same sums as the per-wave kernel on the same draws."""
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batch_pair_kernel_frozen_items_match_per_wave_kernel(dtype):
+    """Items that do not train run the plain event body of their precision
+    inside the pair kernel: same sums as the per-wave kernel on the same
+    draws."""
     specs = pair_specs()
-    a = BatchVegas(specs, dtype=torch.float32, train=False, events_per_tile=4096, blocks_per_cu=4)
-    b = BatchVegas(specs, dtype=torch.float32, train=False, events_per_tile=65536, pair_histograms=True)
+    a = BatchVegas(specs, dtype=dtype, train=False, events_per_tile=4096, blocks_per_cu=4)
+    b = BatchVegas(specs, dtype=dtype, train=False, events_per_tile=65536, pair_histograms=True)
     a.run(2, graph=False)
     b.run(2, graph=False)
     np.testing.assert_allclose(b.history(), a.history(), rtol=1e-10)

@@ -27,11 +27,12 @@ def mixed_requests(n, seed=0):
     return out
 
 
-@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_results_same_alone_and_in_the_mix(dtype):
-    """FP64, and mixed precision through the pair-histogram batch kernel."""
-    server = Server(memory_budget=64 << 20, dtype=dtype)
-    assert server.pairs == (dtype == torch.float32)
+@pytest.mark.parametrize("dtype,pairs", [(torch.float64, None), (torch.float32, None), (torch.float64, True)])
+def test_results_same_alone_and_in_the_mix(dtype, pairs):
+    """FP64, mixed precision (through the pair-histogram batch kernel by
+    default), and FP64 through its pair-histogram batch kernel on request."""
+    server = Server(memory_budget=64 << 20, dtype=dtype, pair_histograms=pairs)
+    assert server.pairs == (dtype == torch.float32 if pairs is None else pairs)
     done = server.run_stream(mixed_requests(18))
     assert len(done) == 18
     for req in done:

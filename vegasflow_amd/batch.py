@@ -2,9 +2,10 @@
 
 Each item is a VegasFlow integral (built-in integrand, n_dim <= 16, its own
 event count and Philox seed). One iteration round is one persistent launch
-of ``mv_batch_fused`` (or, in mixed precision, ``mv_batch_fused_pairs``) per
-integrand family present (workgroups stride over tiles of consecutive events
-of one item) plus one ``mv_batch_finalize``
+of ``mv_batch_fused`` (or the pair-histogram kernel, ``mv_batch_fused_pairs``
+or ``mv_batch_fused_pairs_f64``) per integrand family present (workgroups
+stride over tiles of consecutive events of one item) plus one
+``mv_batch_finalize``
 (sums, sigma, running weighted average, history, grid refinement), with all
 state resident on the device. Item i draws exactly what
 ``VegasFlow(n_dim_i, n_events_i, seed=seed_i)`` would, so its results equal a
@@ -43,10 +44,11 @@ class BatchVegas:
         train: refine every item's grid after each iteration
         max_iter: iterations kept in the per-item history
         events_per_tile, blocks_per_cu: launch shape; autotuned when None
-        pair_histograms: mixed precision only: train through pair tables
-            (mv_batch_fused_pairs, 1024-thread workgroups, as many as are
-            resident; families whose max_dim it does not take fall back to
-            mv_batch_fused with blocks_per_cu); autotuned when None
+        pair_histograms: train through pair tables (mv_batch_fused_pairs,
+            or mv_batch_fused_pairs_f64 in FP64; 1024-thread workgroups, as
+            many as are resident; families whose max_dim it does not take
+            fall back to mv_batch_fused with blocks_per_cu); autotuned when
+            None
     """
 
     def __init__(self, specs, dtype=None, train=True, max_iter=64, events_per_tile=None, blocks_per_cu=None,
@@ -104,9 +106,6 @@ class BatchVegas:
         self._graph = None
         self._warmed = False
 
-        mixed = self.precision == "mixed"
-        if pair_histograms and not mixed:
-            raise ValueError("pair histograms are a mixed-precision kernel")
         if events_per_tile is None or (blocks_per_cu is None and not pair_histograms):
             ept, bpc, pairs = self._tune(pair_histograms)
             events_per_tile = events_per_tile or ept
@@ -114,7 +113,7 @@ class BatchVegas:
             if pair_histograms is None:
                 pair_histograms = bool(pairs)
         self.events_per_tile = int(events_per_tile)
-        self.pairs = bool(pair_histograms) and mixed
+        self.pairs = bool(pair_histograms)
         self.blocks = int(blocks_per_cu or autotune.BLOCKS_PER_CU[-1]) * self._cus
         self._launches = self._tile_launches(self.events_per_tile)
         self._pair_blocks = self._pair_launch_blocks(self._launches) if self.pairs else {}
@@ -142,10 +141,9 @@ class BatchVegas:
             launches.append((fam, dev_tiles, len(tiles), max_dim))
         return launches
 
-    @staticmethod
-    def _pair_launch_blocks(launches):
+    def _pair_launch_blocks(self, launches):
         """Resident pair-kernel workgroups per family launch (0: not taken)."""
-        return {fam: kernels.batch_pairs_blocks(fam, maxd) for fam, _, _, maxd in launches}
+        return {fam: kernels.batch_pairs_blocks(fam, maxd, self.precision) for fam, _, _, maxd in launches}
 
     def _signature(self):
         h = hashlib.sha1()
@@ -158,15 +156,15 @@ class BatchVegas:
             pb = pair_blocks.get(fam, 0)
             if pb > 0:
                 kernels.batch_fused_pairs(self.items, self.n_items, tiles, nt, fam, maxd, self.draws,
-                                          self.grids, rows, pb)
+                                          self.grids, rows, pb, self.precision)
             else:
                 kernels.batch_fused(self.items, self.n_items, tiles, nt, self.precision, fam, maxd,
                                     self.draws, self.grids, rows, blocks)
 
     def _tune(self, pair_histograms=None):
         """Fastest (events per tile, workgroups per CU, pairs) for this batch.
-        pair_histograms None: both kernels compete (mixed only); True / False:
-        only the pair / per-wave candidates."""
+        pair_histograms None: both kernels compete; True / False: only the
+        pair / per-wave candidates."""
         scratch = torch.zeros_like(self.rows)
 
         def measure(cand):
@@ -185,7 +183,7 @@ class BatchVegas:
         cands = []
         if not pair_histograms:
             cands += [(e, b, 0) for e in autotune.EVENTS_PER_TILE for b in autotune.BLOCKS_PER_CU]
-        if self.precision == "mixed" and pair_histograms is not False:
+        if pair_histograms is not False:
             cands += [(e, 0, 1) for e in autotune.PAIR_EVENTS_PER_TILE]
         choice = autotune.tune(self._signature() + f"|pairs={pair_histograms}", cands, measure)
         logger.info("batch launch shape: %d events per tile, %d workgroups per CU, pairs %d", *choice)
@@ -207,7 +205,7 @@ class BatchVegas:
         for fam, tiles, nt, maxd in self._launches:
             if self._pair_blocks.get(fam, 0) > 0:
                 kernels.batch_fused_pairs(self.items, self.n_items, tiles, nt, fam, maxd, draws, grids, rows,
-                                          self._pair_blocks[fam])
+                                          self._pair_blocks[fam], self.precision)
             else:
                 kernels.batch_fused(self.items, self.n_items, tiles, nt, self.precision, fam, maxd, draws,
                                     grids, rows, self.blocks)

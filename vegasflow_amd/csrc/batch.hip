@@ -104,52 +104,78 @@ __global__ __launch_bounds__(kBlock) void batch_fused(const mv_batch_item* __res
   }
 }
 
-// Mixed-precision tiles with pair histograms (fused_event.hpp), BS threads:
-// LDS = the map table (nbmax float2), then pair_cells(max_dim) doubles.
-// Training items add one LDS atomic per pair of dimensions; frozen items
-// run the plain mixed event body.
-template <int FAM, int BS, int D = 0>
+// Tiles with pair histograms (fused_event.hpp), BS threads. LDS = the map
+// table (mixed: nbmax float2; FP64: nbmax (P, Q) pairs and nbmax W), then
+// pair_cells(max_dim) doubles. Training items add one LDS atomic per pair of
+// dimensions; frozen items run the plain event body of their precision.
+template <int FAM, int BS, bool MIXED, int D = 0>
 __device__ __forceinline__ void batch_tile_pairs(const mv_batch_item& it, const mv_batch_tile& tile,
                                                  uint32_t draw, const double* __restrict__ grids,
                                                  double* __restrict__ rows, double* smem, int nbmax) {
   const int d = D ? D : it.n_dim;  // D > 0: compiled for that n_dim (straight-line event body)
-  float2* tab = reinterpret_cast<float2*>(smem);
-  double* H = smem + nbmax;  // 16-byte aligned: nbmax * 8 is a multiple of 400
-  __syncthreads();            // the previous tile is done with the LDS
-  build_table<float, FAM>(it.fn, grids + it.grid_offset, d, nullptr, nullptr, tab, nullptr);
-  if (it.train) {
-    const int half = pair_cells(d) / 2;
-    for (int i = threadIdx.x; i < half; i += BS) reinterpret_cast<double2*>(H)[i] = make_double2(0.0, 0.0);
-  }
-  const float cprod = jacobian_product<FAM>(it.fn, d, nullptr);
-  __syncthreads();
-  Integrand<float, FAM> F(it.fn);
+  // 16-byte aligned: nbmax * 8 is a multiple of 400
+  double* H = smem + (MIXED ? nbmax : 3 * nbmax);
+  __syncthreads();  // the previous tile is done with the LDS
   const Key key{(uint32_t)(it.seed & 0xffffffffu), (uint32_t)(it.seed >> 32)};
   const double wscale = 1.0 / (double)it.n_events;  // xjac, monte_carlo.py:224-227
   // a tile never crosses a multiple of 2^32 events: uniform high counter word
   const uint32_t lo0 = (uint32_t)((uint64_t)tile.first & 0xffffffffu);
   const uint32_t hi = (uint32_t)((uint64_t)tile.first >> 32);
+  const int groups = (d + 3) / 4;
   double acc = 0.0, acc2 = 0.0;
-  if (it.train) {
-    for (int k = threadIdx.x; k < tile.count; k += BS) {
-      const double tmp = event_mixed_pairs<FAM, false>(F, lo0 + (uint32_t)k, hi, draw, key, d, tab,
-                                                       cprod, H, nullptr, wscale);
-      acc += tmp;
-      acc2 += tmp * tmp;
+  if constexpr (MIXED) {
+    float2* tab = reinterpret_cast<float2*>(smem);
+    build_table<float, FAM>(it.fn, grids + it.grid_offset, d, nullptr, nullptr, tab, nullptr);
+    if (it.train) {
+      const int half = pair_cells(d) / 2;
+      for (int i = threadIdx.x; i < half; i += BS) reinterpret_cast<double2*>(H)[i] = make_double2(0.0, 0.0);
+    }
+    const float cprod = jacobian_product<FAM>(it.fn, d, nullptr);
+    __syncthreads();
+    Integrand<float, FAM> F(it.fn);
+    if (it.train) {
+      for (int k = threadIdx.x; k < tile.count; k += BS) {
+        const double tmp = event_mixed_pairs<FAM, false>(F, lo0 + (uint32_t)k, hi, draw, key, d, tab,
+                                                         cprod, H, nullptr, wscale);
+        acc += tmp;
+        acc2 += tmp * tmp;
+      }
+    } else {
+      for (int k = threadIdx.x; k < tile.count; k += BS) {
+        const double tmp = event_mixed<FAM, false>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d, tab,
+                                                   cprod, nullptr, nullptr, wscale);
+        acc += tmp;
+        acc2 += tmp * tmp;
+      }
     }
   } else {
-    const int groups = (d + 3) / 4;
-    for (int k = threadIdx.x; k < tile.count; k += BS) {
-      const double tmp = event_mixed<FAM, false>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d, tab,
-                                                 cprod, nullptr, nullptr, wscale);
-      acc += tmp;
-      acc2 += tmp * tmp;
+    double2* tab = reinterpret_cast<double2*>(smem);
+    double* tw = smem + 2 * nbmax;
+    build_table<double, FAM>(it.fn, grids + it.grid_offset, d, nullptr, nullptr, tab, tw);
+    if (it.train)
+      for (int i = threadIdx.x; i < pair_cells(d); i += BS) H[i] = 0.0;
+    __syncthreads();
+    Integrand<double, FAM> F(it.fn);
+    if (it.train) {
+      for (int k = threadIdx.x; k < tile.count; k += BS) {
+        const double tmp = event_f64<FAM, true, true, 0>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
+                                                          tab, tw, H, nullptr, wscale);
+        acc += tmp;
+        acc2 += tmp * tmp;
+      }
+    } else {
+      for (int k = threadIdx.x; k < tile.count; k += BS) {
+        const double tmp = event_f64<FAM, false, false, 0>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
+                                                            tab, tw, nullptr, nullptr, wscale);
+        acc += tmp;
+        acc2 += tmp * tmp;
+      }
     }
   }
   finish_pairs<BS>(acc, acc2, H, it.train ? d : 0, rows + it.row_offset);
 }
 
-template <int FAM, int BS>
+template <int FAM, int BS, bool MIXED>
 __global__ __launch_bounds__(BS) void batch_fused_pairs(const mv_batch_item* __restrict__ items,
                                                         const mv_batch_tile* __restrict__ tiles,
                                                         int n_tiles, int nbmax,
@@ -163,21 +189,21 @@ __global__ __launch_bounds__(BS) void batch_fused_pairs(const mv_batch_item* __r
     const uint32_t draw = (uint32_t)draws[tile.item];
     if constexpr (FAM >= 0) {
       // one event body per n_dim 1..kPairMaxDim, chosen per tile
-#define MV_BTP(D) case D: batch_tile_pairs<FAM, BS, D>(it, tile, draw, grids, rows, smem, nbmax); break
+#define MV_BTP(D) case D: batch_tile_pairs<FAM, BS, MIXED, D>(it, tile, draw, grids, rows, smem, nbmax); break
       switch (it.n_dim) {
         MV_BTP(1); MV_BTP(2); MV_BTP(3); MV_BTP(4); MV_BTP(5); MV_BTP(6); MV_BTP(7);
         MV_BTP(8); MV_BTP(9); MV_BTP(10); MV_BTP(11); MV_BTP(12); MV_BTP(13); MV_BTP(14);
-        default: batch_tile_pairs<FAM, BS>(it, tile, draw, grids, rows, smem, nbmax); break;
+        default: batch_tile_pairs<FAM, BS, MIXED>(it, tile, draw, grids, rows, smem, nbmax); break;
       }
 #undef MV_BTP
       static_assert(kPairMaxDim == 14, "the switch above covers n_dim 1..14");
     } else {
       switch (it.fn.family) {
-        case MV_FN_CONSTANT: batch_tile_pairs<MV_FN_CONSTANT, BS>(it, tile, draw, grids, rows, smem, nbmax); break;
-        case MV_FN_GAUSSIAN: batch_tile_pairs<MV_FN_GAUSSIAN, BS>(it, tile, draw, grids, rows, smem, nbmax); break;
-        case MV_FN_PRODUCT: batch_tile_pairs<MV_FN_PRODUCT, BS>(it, tile, draw, grids, rows, smem, nbmax); break;
-        case MV_FN_GAUSSMIX: batch_tile_pairs<MV_FN_GAUSSMIX, BS>(it, tile, draw, grids, rows, smem, nbmax); break;
-        default: batch_tile_pairs<MV_FN_BWPOWER, BS>(it, tile, draw, grids, rows, smem, nbmax); break;
+        case MV_FN_CONSTANT: batch_tile_pairs<MV_FN_CONSTANT, BS, MIXED>(it, tile, draw, grids, rows, smem, nbmax); break;
+        case MV_FN_GAUSSIAN: batch_tile_pairs<MV_FN_GAUSSIAN, BS, MIXED>(it, tile, draw, grids, rows, smem, nbmax); break;
+        case MV_FN_PRODUCT: batch_tile_pairs<MV_FN_PRODUCT, BS, MIXED>(it, tile, draw, grids, rows, smem, nbmax); break;
+        case MV_FN_GAUSSMIX: batch_tile_pairs<MV_FN_GAUSSMIX, BS, MIXED>(it, tile, draw, grids, rows, smem, nbmax); break;
+        default: batch_tile_pairs<MV_FN_BWPOWER, BS, MIXED>(it, tile, draw, grids, rows, smem, nbmax); break;
       }
     }
   }
@@ -271,50 +297,48 @@ extern "C" int mv_batch_fused(const mv_batch_item* items, int32_t n_items, const
 namespace {
 constexpr int kBatchPairBlock = 1024;
 
-size_t batch_pairs_lds(int max_dim) {
+size_t batch_pairs_lds(int max_dim, bool mixed) {
   if (max_dim < 1 || max_dim > kPairMaxDim) return 0;
-  const size_t lds = (size_t)(max_dim * kBins + pair_cells(max_dim)) * sizeof(double);
+  const size_t table = (size_t)max_dim * kBins * (mixed ? 1 : 3);
+  const size_t lds = (table + pair_cells(max_dim)) * sizeof(double);
   return lds <= max_lds_per_block() ? lds : 0;
 }
 
-template <int F>
+template <int F, bool MIXED>
 const void* batch_pairs_kernel() {
-  return reinterpret_cast<const void*>(batch_fused_pairs<F, kBatchPairBlock>);
+  return reinterpret_cast<const void*>(batch_fused_pairs<F, kBatchPairBlock, MIXED>);
 }
-}  // namespace
 
-extern "C" int32_t mv_batch_pairs_blocks(int32_t family, int32_t max_dim) {
-  const size_t lds = batch_pairs_lds(max_dim);
+template <bool MIXED>
+int32_t pairs_blocks(int32_t family, int32_t max_dim) {
+  const size_t lds = batch_pairs_lds(max_dim, MIXED);
   if (!lds) return 0;
   const void* k = nullptr;
   switch (family) {
-    case MV_FN_CONSTANT: k = batch_pairs_kernel<MV_FN_CONSTANT>(); break;
-    case MV_FN_GAUSSIAN: k = batch_pairs_kernel<MV_FN_GAUSSIAN>(); break;
-    case MV_FN_PRODUCT: k = batch_pairs_kernel<MV_FN_PRODUCT>(); break;
-    case MV_FN_GAUSSMIX: k = batch_pairs_kernel<MV_FN_GAUSSMIX>(); break;
-    case MV_FN_BWPOWER: k = batch_pairs_kernel<MV_FN_BWPOWER>(); break;
-    default: k = batch_pairs_kernel<-1>(); break;
+    case MV_FN_CONSTANT: k = batch_pairs_kernel<MV_FN_CONSTANT, MIXED>(); break;
+    case MV_FN_GAUSSIAN: k = batch_pairs_kernel<MV_FN_GAUSSIAN, MIXED>(); break;
+    case MV_FN_PRODUCT: k = batch_pairs_kernel<MV_FN_PRODUCT, MIXED>(); break;
+    case MV_FN_GAUSSMIX: k = batch_pairs_kernel<MV_FN_GAUSSMIX, MIXED>(); break;
+    case MV_FN_BWPOWER: k = batch_pairs_kernel<MV_FN_BWPOWER, MIXED>(); break;
+    default: k = batch_pairs_kernel<-1, MIXED>(); break;
   }
   return resident_blocks(k, lds, kBatchPairBlock);
 }
 
-extern "C" int mv_batch_fused_pairs(const mv_batch_item* items, int32_t n_items,
-                                    const mv_batch_tile* tiles, int32_t n_tiles, int32_t family,
-                                    int32_t max_dim, const int32_t* draws, const double* grids,
-                                    double* rows, int32_t blocks, void* stream) {
-  if (n_items < 0 || n_tiles < 0) return fail(MV_ERR_ARG, "mv_batch_fused_pairs: negative counts");
+template <bool MIXED>
+int fused_pairs(const mv_batch_item* items, int32_t n_items, const mv_batch_tile* tiles, int32_t n_tiles,
+                int32_t family, int32_t max_dim, const int32_t* draws, const double* grids, double* rows,
+                int32_t blocks, void* stream, const char* what) {
+  if (n_items < 0 || n_tiles < 0) return fail(MV_ERR_ARG, "%s: negative counts", what);
   if (n_tiles == 0) return MV_OK;
-  if (!items || !tiles || !draws || !grids || !rows)
-    return fail(MV_ERR_ARG, "mv_batch_fused_pairs: missing buffer");
-  const size_t lds = batch_pairs_lds(max_dim);
-  if (!lds)
-    return fail(MV_ERR_UNSUPPORTED, "mv_batch_fused_pairs: max_dim must be 1..%d and fit the LDS",
-                kPairMaxDim);
-  if (blocks < 1) return fail(MV_ERR_ARG, "mv_batch_fused_pairs: blocks < 1");
+  if (!items || !tiles || !draws || !grids || !rows) return fail(MV_ERR_ARG, "%s: missing buffer", what);
+  const size_t lds = batch_pairs_lds(max_dim, MIXED);
+  if (!lds) return fail(MV_ERR_UNSUPPORTED, "%s: max_dim must be 1..%d and fit the LDS", what, kPairMaxDim);
+  if (blocks < 1) return fail(MV_ERR_ARG, "%s: blocks < 1", what);
   const int nbmax = max_dim * kBins;
   hipStream_t s = as_stream(stream);
-#define MV_BP(F)                                                                              \
-  hipLaunchKernelGGL((batch_fused_pairs<F, kBatchPairBlock>), dim3(blocks), dim3(kBatchPairBlock), \
+#define MV_BP(F)                                                                                     \
+  hipLaunchKernelGGL((batch_fused_pairs<F, kBatchPairBlock, MIXED>), dim3(blocks), dim3(kBatchPairBlock), \
                      lds, s, items, tiles, n_tiles, nbmax, draws, grids, rows)
   switch (family) {
     case MV_FN_CONSTANT: MV_BP(MV_FN_CONSTANT); break;
@@ -325,7 +349,32 @@ extern "C" int mv_batch_fused_pairs(const mv_batch_item* items, int32_t n_items,
     default: MV_BP(-1); break;
   }
 #undef MV_BP
-  return check_launch("mv_batch_fused_pairs");
+  return check_launch(what);
+}
+}  // namespace
+
+extern "C" int32_t mv_batch_pairs_blocks(int32_t family, int32_t max_dim) {
+  return pairs_blocks<true>(family, max_dim);
+}
+
+extern "C" int32_t mv_batch_pairs_blocks_f64(int32_t family, int32_t max_dim) {
+  return pairs_blocks<false>(family, max_dim);
+}
+
+extern "C" int mv_batch_fused_pairs(const mv_batch_item* items, int32_t n_items,
+                                    const mv_batch_tile* tiles, int32_t n_tiles, int32_t family,
+                                    int32_t max_dim, const int32_t* draws, const double* grids,
+                                    double* rows, int32_t blocks, void* stream) {
+  return fused_pairs<true>(items, n_items, tiles, n_tiles, family, max_dim, draws, grids, rows, blocks, stream,
+                           "mv_batch_fused_pairs");
+}
+
+extern "C" int mv_batch_fused_pairs_f64(const mv_batch_item* items, int32_t n_items,
+                                        const mv_batch_tile* tiles, int32_t n_tiles, int32_t family,
+                                        int32_t max_dim, const int32_t* draws, const double* grids,
+                                        double* rows, int32_t blocks, void* stream) {
+  return fused_pairs<false>(items, n_items, tiles, n_tiles, family, max_dim, draws, grids, rows, blocks, stream,
+                            "mv_batch_fused_pairs_f64");
 }
 
 extern "C" int mv_batch_finalize(const mv_batch_item* items, const int32_t* jobs, int32_t n_jobs,

@@ -377,17 +377,22 @@ def batch_fused(items_dev, n_items, tiles_dev, n_tiles, precision, family, max_d
               int(blocks), _stream())
 
 
-def batch_fused_pairs(items_dev, n_items, tiles_dev, n_tiles, family, max_dim, draws, grids, rows, blocks):
-    """mv_batch_fused_pairs (mixed precision, pair histograms, 1024-thread
-    workgroups); arguments as batch_fused."""
-    _abi.call("mv_batch_fused_pairs", _ptr(items_dev), int(n_items), _ptr(tiles_dev), int(n_tiles),
+def batch_fused_pairs(items_dev, n_items, tiles_dev, n_tiles, family, max_dim, draws, grids, rows, blocks,
+                      precision="mixed"):
+    """mv_batch_fused_pairs (mixed) or mv_batch_fused_pairs_f64: pair
+    histograms, 1024-thread workgroups; arguments as batch_fused."""
+    name = "mv_batch_fused_pairs" if precision_code(precision) == _abi.MIXED else "mv_batch_fused_pairs_f64"
+    _abi.call(name, _ptr(items_dev), int(n_items), _ptr(tiles_dev), int(n_tiles),
               int(family), int(max_dim), _ptr(draws), _ptr(grids), _ptr(rows), int(blocks), _stream())
 
 
-def batch_pairs_blocks(family, max_dim):
-    """Resident 1024-thread workgroups of mv_batch_fused_pairs on the current
-    device for this family and max_dim (0: the pair kernel does not apply)."""
-    return int(_abi.load().mv_batch_pairs_blocks(int(family), int(max_dim)))
+def batch_pairs_blocks(family, max_dim, precision="mixed"):
+    """Resident 1024-thread workgroups of the pair batch kernel of this
+    precision on the current device for this family and max_dim (0: the pair
+    kernel does not apply)."""
+    lib = _abi.load()
+    fn = lib.mv_batch_pairs_blocks if precision_code(precision) == _abi.MIXED else lib.mv_batch_pairs_blocks_f64
+    return int(fn(int(family), int(max_dim)))
 
 
 def batch_finalize(items_dev, jobs, n_jobs, grids, rows, draws, stats, history=None, max_iter=0):
