@@ -7,14 +7,13 @@ Philox words bit-exact; FP64 sums, sigma, histograms and refined edges
 within 1e-12 relative (the fused kernels may use FMAs; the sampling kernel
 reproduces the reference's op order and is compared at 1e-15).
 """
-import math
 
 import numpy as np
 import pytest
 import torch
 
 from oracle import vegas as ora
-from vegasflow_amd import _abi, kernels
+from vegasflow_amd import kernels
 from vegasflow_amd import integrands as vfi
 
 pytestmark = pytest.mark.gpu

@@ -16,7 +16,6 @@ replayed: the per-item iteration counters live on the device, so nothing in a
 round depends on the host. The tile size, workgroup count and kernel come
 from the autotuner (vegasflow_amd.autotune) unless given.
 """
-import ctypes
 import hashlib
 import logging
 
