@@ -163,6 +163,24 @@ def bench_unfused(vf, dtype, steps=10):
             "hbm_peak_GB_s": 8000.0}
 
 
+def bench_plain(vf, dtype, n_iter=20):
+    """PlainFlow, d=8 Gaussian a=0.1, 1e7 events per iteration: a warm-up run
+    of n_iter iterations (the second half replayed from the captured
+    iteration), then n_iter timed iterations, all replayed."""
+    import torch
+
+    fn = vf.Gaussian(A, 0.5)
+    inst = vf.PlainFlow(D, N, seed=SEED, verbose=False, dtype=dtype)
+    inst.compile(fn)
+    inst.run_integration(n_iter)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res, sig = inst.run_integration(n_iter)
+    dt = time.perf_counter() - t0
+    return {"events_per_s": N * n_iter / dt, "ms_per_iteration": 1e3 * dt / n_iter, "iterations": n_iter,
+            "integral": float(res), "sigma": float(sig), "pull_vs_exact": (float(res) - fn.exact(D)) / float(sig)}
+
+
 def bench_vplus(vf, dtype, n_iter=10):
     """VegasFlowPlus, d=4 two-Gaussian mixture, 1e6 events requested,
     adaptive (5e5 events in the first iteration), 10 iterations."""
@@ -335,7 +353,7 @@ def main():
     ap.add_argument("--no-protocol", action="store_true", help="skip the 5+5 iteration error check")
     ap.add_argument("--no-extra", action="store_true", help="skip the VEGAS+, batched and serving runs")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the Poisson stream")
-    ap.add_argument("--only", choices=["unfused", "vplus", "batch", "serve"],
+    ap.add_argument("--only", choices=["unfused", "plain", "vplus", "batch", "serve"],
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -358,6 +376,8 @@ def main():
             out["measured_copy_GB_s"] = calibrate_copy(torch)["GB_per_s_read_plus_write"]
         elif args.only == "vplus":
             out = {p: bench_vplus(vf, dt_) for p, dt_ in precisions}
+        elif args.only == "plain":
+            out = {p: bench_plain(vf, dt_) for p, dt_ in precisions}
         elif args.only == "batch":
             out = {p: bench_batched(torch, dt_) for p, dt_ in precisions}
         else:
@@ -450,6 +470,8 @@ def main():
         # the achievable HBM rate next to the 8 TB/s datasheet figure
         line["unfused_torch_integrand"]["measured_copy_GB_s"] = \
             calibrate_copy(torch)["GB_per_s_read_plus_write"]
+        line["plainflow_d8"] = {p: bench_plain(vf, dt_) for p, dt_ in
+                                (("mixed", torch.float32), ("f64", torch.float64))}
         line["vegasflowplus_d4"] = {p: bench_vplus(vf, dt_) for p, dt_ in
                                     (("mixed", torch.float32), ("f64", torch.float64))}
         line["batched_256"] = {p: bench_batched(torch, dt_) for p, dt_ in

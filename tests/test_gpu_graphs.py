@@ -74,6 +74,21 @@ def test_short_first_runs_stay_eager():
     assert len(inst._graphs) == 1 and inst._draw_count == 5 + inst.graph_min_iterations
 
 
+def test_large_iterations_stay_eager():
+    """Iterations above graph_max_events run eagerly even in long runs, and
+    give the same results as a replayed instance with the limit raised."""
+    big = vf.VegasFlow(4, 50_000, seed=6, verbose=False)
+    big.graph_max_events = 10_000
+    big.compile(vf.Gaussian(0.2, 0.5))
+    r1 = big.run_integration(big.graph_min_iterations)
+    assert not big._graphs
+    small = vf.VegasFlow(4, 50_000, seed=6, verbose=False)
+    small.compile(vf.Gaussian(0.2, 0.5))
+    r2 = small.run_integration(small.graph_min_iterations)
+    assert small._graphs
+    assert abs(r1[0] - r2[0]) <= 1e-12 * abs(r2[0]) and abs(r1[1] - r2[1]) <= 1e-9 * abs(r2[1])
+
+
 def test_vegasflow_replay_with_limits_and_single_iterations():
     eager, graph = _pair(vf.VegasFlow, 3, 50_000, seed=3, xmin=[0.0, -1.0, 2.0], xmax=[1.0, 1.0, 3.5])
     fn = vf.Product()
