@@ -329,3 +329,122 @@ def test_parallel_cut_form_equals_serial_rebin():
         a = ora.refine_grid_per_dimension(h, div)
         b = _cuts_by_prefix_counts(h, div)
         np.testing.assert_allclose(b, a, rtol=1e-13, atol=1e-16)
+
+
+# ------------------------------------------------- integrand closed forms
+def test_breit_wigner_power_separable_closed_forms():
+    """oracle.breit_wigner_power against integrals it does not contain:
+    quadrature of its x_0 factor (at x_1 = 1) is the normalised Cauchy mass
+    on [0, 1], (atan(0.5/G2) - atan(-0.5/G2))/pi with G2 = width/2 = 0.01;
+    quadrature of its x_1 factor (at x_0 = M, divided by the peak value) is
+    int_0^1 x^-0.7 = 10/3; the function is flat in the other dimensions and
+    factorises, so its integral is the product 3.2908976726601815."""
+    from scipy import integrate
+
+    f = ora.breit_wigner_power
+    g2 = 0.01
+    bw_mass = (math.atan(0.5 / g2) - math.atan(-0.5 / g2)) / math.pi
+    assert abs(bw_mass - 0.98726930179805439) < 1e-15
+    i0, _ = integrate.quad(lambda t: f(np.array([[t, 1.0, 0.3]]))[0], 0.0, 1.0, points=[0.5],
+                           epsabs=0.0, epsrel=1e-13, limit=200)
+    assert abs(i0 - bw_mass) < 1e-13
+    peak = f(np.array([[0.5, 1.0, 0.3]]))[0]
+    assert abs(peak - 1.0 / (math.pi * g2)) < 1e-12 * peak
+    i1, _ = integrate.quad(lambda t: f(np.array([[0.5, t, 0.3]]))[0] / peak, 0.0, 1.0,
+                           epsabs=0.0, epsrel=1e-12, limit=200)
+    assert abs(i1 - 10.0 / 3.0) < 1e-11
+    assert abs(bw_mass * 10.0 / 3.0 - 3.2908976726601815) < 1e-15
+    # factorisation and flatness at random points
+    rng = np.random.default_rng(6)
+    x = rng.uniform(1e-6, 1.0, size=(64, 5))
+    ref = np.array([[0.5, 1.0, 0.3, 0.3, 0.3]])
+    fx = f(x)
+    x0_only = np.repeat(ref, 64, axis=0)
+    x0_only[:, 0] = x[:, 0]
+    x1_only = np.repeat(ref, 64, axis=0)
+    x1_only[:, 1] = x[:, 1]
+    np.testing.assert_allclose(fx, f(x0_only) * f(x1_only) / peak, rtol=1e-13)
+    # another power: int x^p = 1/(1 + p)
+    i1b, _ = integrate.quad(lambda t: ora.breit_wigner_power(np.array([[0.5, t]]), power=-0.3)[0] / peak,
+                            0.0, 1.0, epsabs=0.0, epsrel=1e-12, limit=200)
+    assert abs(i1b - 1.0 / 0.7) < 1e-11
+
+
+def test_gaussian_and_mixture_closed_forms():
+    """oracle.gaussian / gaussian_mix against products of 1-d quadratures
+    (the Gaussian factorises over dimensions)."""
+    from scipy import integrate
+
+    for a, c in [(0.1, 0.5), (0.2, 0.37), (0.05, 0.33)]:
+        one, _ = integrate.quad(lambda t: ora.gaussian(np.array([[t]]), a, c)[0], 0.0, 1.0,
+                                points=[c], epsabs=0.0, epsrel=1e-13, limit=200)
+        assert abs(one - 0.5 * (math.erf((1 - c) / a) + math.erf(c / a))) < 1e-13
+        # d = 3: product of the 1-d factors at a random point
+        x = np.array([[0.41, 0.52, 0.6]])
+        got = ora.gaussian(x, a, c)[0]
+        want = np.prod([ora.gaussian(np.array([[t]]), a, c)[0] for t in x[0]])
+        assert abs(got - want) <= 1e-13 * want
+    # C3 mixture, normalised like C1: 0.9999969422972157
+    exact = 0.0
+    for wk, ak, ck in [(0.5, 0.05, 0.33), (0.5, 0.1, 0.67)]:
+        exact += wk * (0.5 * (math.erf((1 - ck) / ak) + math.erf(ck / ak))) ** 4
+    assert abs(exact - 0.9999969422972157) < 1e-15
+
+
+# ------------------------------------------------------ sigma pull tests
+def _pull_moments(pulls):
+    p = np.asarray(pulls)
+    return float(p.mean()), float(p.std())
+
+
+def _assert_unit_pulls(res, sig, exact):
+    """(res - exact)/sigma over independent seeds has mean ~0 and standard
+    deviation ~1 when sigma estimates the error of res; a sigma off by a
+    factor (the wrong N - 1, a missing square) fails the band."""
+    res = np.asarray(res)
+    sig = np.asarray(sig)
+    mean, std = _pull_moments((res - exact) / sig)
+    n = res.size
+    assert abs(mean) < 4.0 / math.sqrt(n), mean
+    assert 0.85 <= std <= 1.15, std
+    # the band rejects a sigma too large or too small by 1.5x
+    for factor in (1.5, 1.0 / 1.5):
+        assert not 0.85 <= _pull_moments((res - exact) / (sig * factor))[1] <= 1.15
+
+
+def test_vegas_sigma_pull_distribution():
+    """vflow.py:437-438 on an adapted (non-uniform) grid, 400 seeds."""
+    fn = lambda x: ora.gaussian(x, 0.3, 0.5)  # noqa: E731
+    exact = math.erf(0.5 / 0.3) ** 2
+    _, grid = ora.run_vegas(fn, 2, 20000, 3, seed=0)
+    assert np.abs(grid - ora.initial_grid(2)).max() > 1e-3  # the grid did adapt
+    n = 2000
+    out = [ora.vegas_iteration(grid, ora.uniform_f64(1000 + s, 0, n, 2), fn, n, train=False)[:2]
+           for s in range(400)]
+    _assert_unit_pulls([o[0] for o in out], [o[1] for o in out], exact)
+
+
+def test_plain_sigma_pull_distribution():
+    """plain.py:37-43, 400 seeds."""
+    fn = lambda x: ora.gaussian(x, 0.3, 0.45)  # noqa: E731
+    exact = (0.5 * (math.erf(0.55 / 0.3) + math.erf(0.45 / 0.3))) ** 3
+    n = 2000
+    out = [ora.plain_iteration(ora.uniform_f64(2000 + s, 0, n, 3), fn, n) for s in range(400)]
+    _assert_unit_pulls([o[0] for o in out], [o[1] for o in out], exact)
+
+
+def test_vplus_sigma_pull_distribution():
+    """vflowplus.py:222-233 with adapted per-cube event counts (first
+    iteration's reallocation), 300 seeds."""
+    fn = lambda x: ora.gaussian(x, 0.3, 0.5)  # noqa: E731
+    exact = math.erf(0.5 / 0.3) ** 2
+    st = ora.vplus_setup(2, 8000, adaptive=True)
+    grid = ora.initial_grid(2)
+    u = ora.uniform_f64(1, 0, int(st["n_ev"].sum()), 2)
+    _, _, grid, new_n_ev, *_ = ora.vplus_iteration(grid, st, u, fn)
+    st["n_ev"] = new_n_ev
+    assert new_n_ev.min() != new_n_ev.max()  # uneven allocation
+    tot = int(new_n_ev.sum())
+    out = [ora.vplus_iteration(grid, st, ora.uniform_f64(3000 + s, 0, tot, 2), fn, train=False)[:2]
+           for s in range(300)]
+    _assert_unit_pulls([o[0] for o in out], [o[1] for o in out], exact)
