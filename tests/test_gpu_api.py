@@ -16,6 +16,7 @@ import torch
 
 import vegasflow_amd as vf
 from oracle import vegas as ora
+from tests.parity import RTOL, assert_sigma, kappa_vegas
 from vegasflow_amd import PlainFlow, VegasFlow, VegasFlowPlus, plain_sampler, vegas_sampler
 
 pytestmark = pytest.mark.gpu
@@ -238,16 +239,36 @@ def test_integration_limits(alg, fused, ncalls=int(1e4)):
 @pytest.mark.parametrize("d", [4, 8])
 def test_vegasflow_iterations_match_oracle(d):
     """Three training iterations through the API (fused kernels, device-side
-    refinement) against the oracle's iterations on the same Philox draws."""
+    refinement) against the oracle on the same Philox draws.
+
+    Per iteration, at the 1e-12 bar: iteration i of the engine equals the
+    oracle's iteration i started from the engine's own grid i (integral,
+    sigma with its cancellation factor, refined edges). Compounded over the
+    run (the oracle's own three iterations from the linspace grid): the
+    engine's grid differs from the oracle's after iteration 1 by the
+    rounding of the histogram sums (FP64 atomics add in arrival order);
+    rebinning turns that into edge shifts of ~1e-15, which move every
+    later point and weight. The bound is the one-iteration bar times the
+    measured growth factor (below 100 for three iterations)."""
     n, seed = 50000, 77
+    fn = lambda x: ora.gaussian(x, 0.1, 0.5)  # noqa: E731
     inst = VegasFlow(d, n, seed=seed, verbose=False)
     inst.compile(vf.Gaussian(0.1, 0.5))
-    inst.run_integration(3)
-    hist, div = ora.run_vegas(lambda x: ora.gaussian(x, 0.1, 0.5), d, n, 3, seed)
+    grids = [inst.divisions.cpu().numpy()]
+    for _ in range(3):
+        inst.run_integration(1)
+        grids.append(inst.divisions.cpu().numpy())
+    np.testing.assert_array_equal(grids[0], ora.initial_grid(d))
+    for i, (r, s, _) in enumerate(inst.history):
+        res, sig, _, o_grid, *_ = ora.vegas_iteration(grids[i], ora.uniform_f64(seed, i, n, d), fn, n)
+        assert abs(r - res) <= RTOL * abs(res)
+        assert_sigma(s, sig, kappa_vegas(res, sig, n))
+        np.testing.assert_allclose(grids[i + 1], o_grid, rtol=RTOL, atol=1e-15)
+    hist, div = ora.run_vegas(fn, d, n, 3, seed)
     for (r, s, _), (orr, os_) in zip(inst.history, hist):
-        assert abs(r - orr) <= 1e-10 * abs(orr)
-        assert abs(s - os_) <= 1e-8 * abs(os_)
-    np.testing.assert_allclose(inst.divisions.cpu().numpy(), div, rtol=1e-10, atol=1e-13)
+        assert abs(r - orr) <= 100 * RTOL * abs(orr)
+        assert_sigma(s, os_, kappa_vegas(orr, os_, n), rtol=100 * RTOL)
+    np.testing.assert_allclose(grids[-1], div, rtol=100 * RTOL, atol=1e-14)
 
 
 def test_fused_and_unfused_paths_agree():
@@ -273,8 +294,8 @@ def test_plainflow_matches_oracle():
     inst.run_integration(2)
     for i, (r, s, _) in enumerate(inst.history):
         orr, os_ = ora.plain_iteration(ora.uniform_f64(seed, i, n, d), ora.product, n)
-        assert abs(r - orr) <= 1e-12 * abs(orr)
-        assert abs(s - os_) <= 1e-9 * abs(os_)
+        assert abs(r - orr) <= RTOL * abs(orr)
+        assert_sigma(s, os_, kappa_vegas(orr, os_, n))
 
 
 def test_mixed_precision_instance():

@@ -13,6 +13,7 @@ import pytest
 import torch
 
 from oracle import vegas as ora
+from tests.parity import assert_sigma, kappa_vegas
 from vegasflow_amd import kernels
 from vegasflow_amd import integrands as vfi
 
@@ -214,7 +215,7 @@ def test_fused_f64_iteration_matches_oracle(fam, d):
     res, sig, arr, o_grid, *_ = ora.vegas_iteration(grid, u, FAMILIES[fam][1], n)
     assert rel(out[0], res) < RTOL
     if sig > 0:
-        assert rel(out[1], sig) < 1e-10  # sigma subtracts two close sums
+        assert_sigma(out[1], sig, kappa_vegas(res, sig, n))
     assert rel(hist, arr) < RTOL
     np.testing.assert_allclose(new_grid, o_grid, rtol=RTOL, atol=1e-15)
 
@@ -302,7 +303,7 @@ def test_sample_digest_path_matches_oracle(d):
     res, sig, arr, o_grid, *_ = ora.vegas_iteration(grid, u, lambda xx: ora.gaussian(xx, 0.3, 0.4), n)
     o = out.cpu().numpy()
     assert rel(o[0], res) < RTOL
-    assert rel(o[1], sig) < 1e-10
+    assert_sigma(o[1], sig, kappa_vegas(res, sig, n))
     assert rel(hist.cpu().numpy(), arr) < RTOL
     np.testing.assert_allclose(g.cpu().numpy(), o_grid, rtol=RTOL, atol=1e-15)
 
@@ -378,7 +379,7 @@ def test_plain_fused_matches_oracle(fam, d):
     out = kernels.vegas_finalize(d, 1, False, False, n, ws, ng).cpu().numpy()
     res, sig = ora.plain_iteration(ora.uniform_f64(21, 2, n, d), ofn, n)
     assert rel(out[0], res) < RTOL
-    assert rel(out[1], sig) < 1e-10
+    assert_sigma(out[1], sig, kappa_vegas(res, sig, n))
 
 
 # ----------------------------------------------------------------- mixed

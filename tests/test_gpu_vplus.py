@@ -13,6 +13,7 @@ import pytest
 import torch
 
 from oracle import vegas as ora
+from tests.parity import assert_sigma, kappa_vplus
 from vegasflow_amd import kernels
 from vegasflow_amd import integrands as vfi
 
@@ -46,6 +47,15 @@ MIX = vfi.GaussianMix((0.5, 0.5), (0.05, 0.1), (0.33, 0.67))
 
 def mix_np(x):
     return ora.gaussian_mix(x, (0.5, 0.5), (0.05, 0.1), (0.33, 0.67))
+
+
+def _oracle_cube_sums(u, st, grid, d):
+    """The oracle's per-cube sums ress and arr_var (vflowplus.py:209-217)."""
+    x, w, ind, seg = ora.generate_samples_in_hypercubes(
+        u, st["n_strat"], st["n_ev"], ora.hypercubes(st["n_strat"], d), grid)
+    x, w = ora.apply_xjac_and_limits(x, w, 1.0 / st["n_cubes"])
+    ress, arr_var, _ = ora.vplus_event(x, w, ind, seg, st["n_ev"], mix_np(x), False)
+    return ress, arr_var
 
 
 def _setup(d, n, adaptive, rng, ragged=True, n_strat=None):
@@ -117,16 +127,13 @@ def test_vplus_fused_iteration_matches_oracle(d, n, n_strat, adaptive):
     res, sig, new_div, new_n_ev, o_var, o_arr = ora.vplus_iteration(grid, st, u, mix_np)
     o = out.cpu().numpy()
     assert rel(o[0], res) < 1e-12
-    assert rel(o[1], sig) < 1e-10
     assert rel(hist.cpu().numpy(), o_arr) < 1e-12
     np.testing.assert_allclose(g.cpu().numpy(), new_div, rtol=1e-12, atol=1e-15)
     # arr_var = ress2 * n_ev - ress^2 cancels where f is nearly flat over a
     # cube (narrow cubes, e.g. 7500 of them at d = 1): the summation-order
     # rounding of ress (a few ulp) then scales with ress^2, not with arr_var
-    x_o, w_o, ind_o, seg_o = ora.generate_samples_in_hypercubes(
-        u, st["n_strat"], st["n_ev"], ora.hypercubes(st["n_strat"], d), grid)
-    x_o, w_o = ora.apply_xjac_and_limits(x_o, w_o, 1.0 / st["n_cubes"])
-    o_ress, _, _ = ora.vplus_event(x_o, w_o, ind_o, seg_o, st["n_ev"], mix_np(x_o), False)
+    o_ress, _ = _oracle_cube_sums(u, st, grid, d)
+    assert_sigma(o[1], sig, kappa_vplus(o_ress, o_var, st["n_ev"], sig))
     cancel = np.square(o_ress)
     scale = np.abs(o_var).max()
     got_var = arr_var.cpu().numpy()
@@ -304,7 +311,8 @@ def test_vplus_fused_counter_crosses_2_32():
     res, sig, new_div, _, _, _ = ora.vplus_iteration(grid, st, u, mix_np)
     o = out.cpu().numpy()
     assert rel(o[0], res) < 1e-12
-    assert rel(o[1], sig) < 1e-10
+    o_ress, o_var = _oracle_cube_sums(u, st, grid, d)
+    assert_sigma(o[1], sig, kappa_vplus(o_ress, o_var, st["n_ev"], sig))
     np.testing.assert_allclose(g.cpu().numpy(), new_div, rtol=1e-12, atol=1e-15)
 
 

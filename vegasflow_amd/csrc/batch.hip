@@ -23,7 +23,7 @@ namespace mv {
 template <bool MIXED, int FAM, bool TRAIN>
 __device__ __forceinline__ void batch_events(const mv_batch_item& it, const mv_batch_tile& tile,
                                              uint32_t draw, const void* tab, const void* tw,
-                                             float cprod, double* hist, int nb, double* row) {
+                                             const JacGroups& cs, double* hist, int nb, double* row) {
   using T = typename std::conditional<MIXED, float, double>::type;
   Integrand<T, FAM> F(it.fn);
   const Key key{(uint32_t)(it.seed & 0xffffffffu), (uint32_t)(it.seed >> 32)};
@@ -40,7 +40,7 @@ __device__ __forceinline__ void batch_events(const mv_batch_item& it, const mv_b
     double tmp;
     if constexpr (MIXED)
       tmp = event_mixed<FAM, TRAIN>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
-                                    static_cast<const float2*>(tab), cprod, wh, nullptr, wscale);
+                                    static_cast<const float2*>(tab), cs, wh, nullptr, wscale);
     else
       tmp = event_f64<FAM, TRAIN>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d,
                                   static_cast<const double2*>(tab), static_cast<const double*>(tw), wh,
@@ -67,13 +67,13 @@ __device__ __forceinline__ void batch_tile(const mv_batch_item& it, const mv_bat
   build_table<T, FAM>(it.fn, grids + it.grid_offset, d, nullptr, nullptr, tab, tw);
   if (it.train)
     for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
-  const float cprod = MIXED ? jacobian_product<FAM>(it.fn, d, nullptr) : 0.0f;
+  const JacGroups cs = MIXED ? jacobian_groups<FAM>(it.fn, d, nullptr) : JacGroups{};
   __syncthreads();
   double* row = rows + it.row_offset;
   if (it.train)
-    batch_events<MIXED, FAM, true>(it, tile, draw, tab, tw, cprod, hist, nb, row);
+    batch_events<MIXED, FAM, true>(it, tile, draw, tab, tw, cs, hist, nb, row);
   else
-    batch_events<MIXED, FAM, false>(it, tile, draw, tab, tw, cprod, hist, nb, row);
+    batch_events<MIXED, FAM, false>(it, tile, draw, tab, tw, cs, hist, nb, row);
 }
 
 // FAM >= 0: every tile belongs to an item of that family (smaller register
@@ -130,20 +130,20 @@ __device__ __forceinline__ void batch_tile_pairs(const mv_batch_item& it, const 
       const int half = pair_cells(d) / 2;
       for (int i = threadIdx.x; i < half; i += BS) reinterpret_cast<double2*>(H)[i] = make_double2(0.0, 0.0);
     }
-    const float cprod = jacobian_product<FAM>(it.fn, d, nullptr);
+    const JacGroups cs = jacobian_groups<FAM>(it.fn, d, nullptr);
     __syncthreads();
     Integrand<float, FAM> F(it.fn);
     if (it.train) {
       for (int k = threadIdx.x; k < tile.count; k += BS) {
         const double tmp = event_mixed_pairs<FAM, false>(F, lo0 + (uint32_t)k, hi, draw, key, d, tab,
-                                                         cprod, H, nullptr, wscale);
+                                                         cs, H, nullptr, wscale);
         acc += tmp;
         acc2 += tmp * tmp;
       }
     } else {
       for (int k = threadIdx.x; k < tile.count; k += BS) {
         const double tmp = event_mixed<FAM, false>(F, lo0 + (uint32_t)k, hi, draw, key, groups, d, tab,
-                                                   cprod, nullptr, nullptr, wscale);
+                                                   cs, nullptr, nullptr, wscale);
         acc += tmp;
         acc2 += tmp * tmp;
       }

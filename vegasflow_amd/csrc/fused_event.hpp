@@ -7,8 +7,9 @@
 // table of the mixed path takes 1 + frac (mixed_position), so it stores
 // (P - Q, Q), and has no W column: W = Q * c_j with the per-dimension
 // constant c_j = 50 / (xdelta_j * fold scale_j) (jacobian_scale), so an
-// event multiplies its slopes Q_j and then the product of the c_j once,
-// instead of one more LDS read per dimension.
+// event multiplies its slopes Q_j and, after each group of four dimensions,
+// the product of that group's c_j (jacobian_groups), instead of one more LDS
+// read per dimension.
 #pragma once
 
 #include <type_traits>
@@ -41,19 +42,36 @@ __device__ __forceinline__ void build_table(const mv_integrand& fn, const double
   }
 }
 
-// c_j of the mixed path's Jacobian: W = dx * 50 = Q * c_j.
+// c_j of the mixed path's Jacobian: W = dx * 50 = Q * c_j. A dimension of
+// zero width (xmin == xmax, allowed by monte_carlo.py:171) has Q = 0 and
+// takes c_j = 0, so its events weigh 0 as in FP64 (w * prod(xdelta) = 0).
 template <int FAM>
-__device__ __forceinline__ float jacobian_scale(const mv_integrand& fn, int j, const double* xdelta) {
+__device__ __forceinline__ double jacobian_scale(const mv_integrand& fn, int j, const double* xdelta) {
   const double xd = xdelta ? xdelta[j] : 1.0;
-  return (float)(kFBins / (xd * Integrand<float, FAM>::fold_scale(fn, j)));
+  if (xd == 0.0) return 0.0;
+  return kFBins / (xd * Integrand<float, FAM>::fold_scale(fn, j));
 }
 
-// prod_j c_j over the n_dim dimensions, in order: the mixed kernels
-// multiply the slopes Q_j of an event and apply this product once.
+// Products of c_j per group of four dimensions (g[k] for dimensions 4k ..
+// 4k + 3), each formed in FP64 and rounded once. An event multiplies its
+// slopes Q_j and, after each group, that group's product, so the running
+// FP32 product never strays from the Jacobian prod (dx_j * 50) of the
+// dimensions done by more than one group's factor. (One product of all the
+// c_j overflows FP32 with narrow limits: 500^16 on [0, 0.1]^16, against a
+// Jacobian of order 1.) Exact in range while every |log2 c_j| < 30.
+struct JacGroups {
+  float g[4];
+};
+
 template <int FAM>
-__device__ __forceinline__ float jacobian_product(const mv_integrand& fn, int d, const double* xdelta) {
-  float c = 1.0f;
-  for (int j = 0; j < d; ++j) c = c * jacobian_scale<FAM>(fn, j, xdelta);
+__device__ __forceinline__ JacGroups jacobian_groups(const mv_integrand& fn, int d, const double* xdelta) {
+  JacGroups c;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double p = 1.0;
+    for (int j = 4 * k; j < 4 * k + 4 && j < d; ++j) p *= jacobian_scale<FAM>(fn, j, xdelta);
+    c.g[k] = (float)p;
+  }
   return c;
 }
 
@@ -160,7 +178,7 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
 template <int FAM, bool TRAIN, bool PAIRS = false, int INJ = -1>
 __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
                                               uint32_t draw, Key key, int groups, int d,
-                                              const float2* tab, float cprod, double* wh,
+                                              const float2* tab, const JacGroups& cs, double* wh,
                                               const uint32_t* inj, double wscale) {
   const bool injected = INJ < 0 ? inj != nullptr : INJ > 0;
   F.begin();
@@ -183,11 +201,12 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
           if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
         }
       }
+      w = w * cs.g[g];
     }
   }
   // w * f in FP32; the 1/N (and limits) factor in FP64 so that its rounding
   // is not a systematic bias
-  const double tmp = (double)((w * cprod) * F.value()) * wscale;
+  const double tmp = (double)(w * F.value()) * wscale;
   if (TRAIN) event_hist<PAIRS>(wh, pk, d, tmp * tmp);
   return tmp;
 }
@@ -226,7 +245,7 @@ __device__ __forceinline__ void pair_dim(Integrand<float, FAM>& F, int j, uint32
 template <int FAM, bool INJ>
 __device__ __forceinline__ double event_mixed_pairs(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
                                                     uint32_t draw, Key key, int d, const float2* tab,
-                                                    float cprod, double* H, const uint32_t* inj,
+                                                    const JacGroups& cs, double* H, const uint32_t* inj,
                                                     double wscale) {
   F.begin();
   float w = 1.0f;
@@ -255,9 +274,10 @@ __device__ __forceinline__ double event_mixed_pairs(Integrand<float, FAM>& F, ui
         for (int q = 0; q < 3; ++q)
           if (4 * g + q < kPairMaxDim && 4 * g + q < d) pair_dim<FAM>(F, 4 * g + q, word(h, q), tab, w, cell);
       }
+      w = w * cs.g[g];
     }
   }
-  const double tmp = (double)((w * cprod) * F.value()) * wscale;
+  const double tmp = (double)(w * F.value()) * wscale;
   const double tmp2 = tmp * tmp;
   char* Hb = reinterpret_cast<char*>(H);
 #pragma unroll

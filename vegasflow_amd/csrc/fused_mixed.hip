@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   build_table<float, FAM>(fn, grid, D, s.xmin, s.xdelta, tab, nullptr);
   if (TRAIN)
     for (int i = threadIdx.x; i < kWaves * nb; i += kBlock) hist[i] = 0.0;
-  const float cprod = jacobian_product<FAM>(fn, D, s.xdelta);
+  const JacGroups cs = jacobian_groups<FAM>(fn, D, s.xdelta);
   __syncthreads();
   double* wh = hist + (threadIdx.x >> 6) * nb;
   Integrand<float, FAM> F(fn);
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
     const double tmp = event_mixed<FAM, TRAIN, false, INJ ? 1 : 0>(
-        F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key, (D + 3) / 4, D, tab, cprod, wh,
+        F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key, (D + 3) / 4, D, tab, cs, wh,
         INJ ? inj + i * D : nullptr, wscale);
     acc += tmp;
     acc2 += tmp * tmp;

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kPairBlock) void vegas_fused_pairs(StepK s, mv_inte
   constexpr int half = pair_cells(D) / 2;
   for (int i = threadIdx.x; i < half; i += kPairBlock)
     reinterpret_cast<double2*>(H)[i] = make_double2(0.0, 0.0);
-  const float cprod = jacobian_product<FAM>(fn, D, s.xdelta);
+  const JacGroups cs = jacobian_groups<FAM>(fn, D, s.xdelta);
   __syncthreads();
   Integrand<float, FAM> F(fn);
   const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kPairBlock) void vegas_fused_pairs(StepK s, mv_inte
   const int64_t i0 = (int64_t)blockIdx.x * kPairBlock + threadIdx.x;
   if (inj) {
     for (int64_t i = i0; i < s.n_events; i += step) {
-      const double tmp = event_mixed_pairs<FAM, true>(F, 0u, 0u, 0u, s.key, D, tab, cprod, H,
+      const double tmp = event_mixed_pairs<FAM, true>(F, 0u, 0u, 0u, s.key, D, tab, cs, H,
                                                       inj + i * D, wscale);
       acc += tmp;
       acc2 += tmp * tmp;
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kPairBlock) void vegas_fused_pairs(StepK s, mv_inte
   } else {
     for (int64_t i = i0; i < s.n_events; i += step) {
       const double tmp = event_mixed_pairs<FAM, false>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw,
-                                                       s.key, D, tab, cprod, H, nullptr, wscale);
+                                                       s.key, D, tab, cs, H, nullptr, wscale);
       acc += tmp;
       acc2 += tmp * tmp;
     }

@@ -70,7 +70,8 @@ __device__ __forceinline__ uint32_t draw_of(const StepK& s) {
 // cross a multiple of 2^32 in the event counter (a single piece unless the
 // range crosses one), advancing event_offset and the injected draws; ``first``
 // is the index of the piece's first event within the step (for outputs).
-// Returns the largest value ``launch`` returned (its workgroup count).
+// Returns the largest value ``launch`` returned (its workgroup count), or
+// the first negative status it returned (no later piece is launched).
 template <typename F>
 int for_each_counter_range(const mv_step& step, size_t inject_elem_bytes, F&& launch) {
   mv_step piece = step;
@@ -81,6 +82,7 @@ int for_each_counter_range(const mv_step& step, size_t inject_elem_bytes, F&& la
     const int64_t room = (int64_t)((1ull << 32) - lo);
     piece.n_events = left < room ? left : room;
     const int b = launch(make_stepk(piece), step.n_events - left);
+    if (b < 0) return b;
     most = b > most ? b : most;
     left -= piece.n_events;
     if (piece.inject)

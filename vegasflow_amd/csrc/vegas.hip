@@ -581,6 +581,7 @@ extern "C" int mv_vegas_fused(const mv_step* step, const mv_integrand* fn, const
       default: return launch_vegas_fused<MV_FN_BWPOWER>(k, *fn, mixed, train, grid, R, st);
     }
   });
+  if (blocks < 0) return blocks;
   if ((rc = check_launch("mv_vegas_fused"))) return rc;
   return rows_used(blocks);
 }
@@ -609,6 +610,7 @@ extern "C" int mv_plain_fused(const mv_step* step, const mv_integrand* fn, const
       default: return launch_plain_fused<MV_FN_BWPOWER>(k, *fn, mixed, R, st);
     }
   });
+  if (blocks < 0) return blocks;
   if ((rc = check_launch("mv_plain_fused"))) return rc;
   return rows_used(blocks);
 }

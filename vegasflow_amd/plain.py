@@ -13,16 +13,20 @@ class PlainFlow(MonteCarloFlow):
     _CAN_RUN_VECTORIAL = True
 
     def _can_fuse(self):
-        return type(self)._run_event is PlainFlow._run_event
+        return self._hooks_native(PlainFlow, ("_run_event", "_digest_random_generation", "_generate_random_array"))
+
+    def _replay_ok(self):
+        return self._hooks_native(PlainFlow, ("_run_iteration",))
 
     def _run_event(self, integrand, ncalls=None):
         """plain.py:18-35: res, raw res2 of one step."""
         n_events = self.n_events if ncalls is None else ncalls
-        rnds, xjac = self._generate_random_array(n_events)
+        rnds, xjac, *_ = self._generate_random_array(n_events)
         f = integrand(rnds, weight=xjac)
         if f.dtype != self._dtype:
             f = f.to(self._dtype)
         f = f.contiguous()
+        xjac = torch.as_tensor(xjac, device=self.device).to(self._dtype).reshape(-1).contiguous()
         n_out = 1 if f.dim() == 1 else f.shape[1]
         ws = self._workspace_for(kernels.partial_stride(self.n_dim, n_out, False))
         step = self._step(n_events, train=False, offset=0)

@@ -17,7 +17,7 @@ import torch
 
 from . import kernels
 from .configflow import BETA, BINS_MAX, MAX_NEVAL_HCUBE
-from .monte_carlo import MonteCarloFlow, sampler, wrapper
+from .monte_carlo import MonteCarloFlow, native_hook, sampler, wrapper
 from .vflow import VegasFlow
 
 logger = logging.getLogger(__name__)
@@ -114,10 +114,13 @@ class VegasFlowPlus(VegasFlow):
     def _strata(self):
         return kernels.make_strata(self._n_strat, self._n_cubes, self._n_ev, self._offsets)
 
+    _UNFUSED_HOOKS = VegasFlow._UNFUSED_HOOKS + ("redistribute_samples",)
+
     def _can_fuse(self):
-        cls = type(self)
-        return (cls._run_event is VegasFlowPlus._run_event
-                and cls._iteration_content is VegasFlowPlus._iteration_content)
+        return self._hooks_native(VegasFlowPlus, self._UNFUSED_HOOKS)
+
+    def _replay_ok(self):
+        return self._hooks_native(VegasFlowPlus, ("_run_iteration", "refine_grid"))
 
     def redistribute_samples(self, arr_var):
         """vflowplus.py:153-163 on the device. Negative variances (rounding)
@@ -135,13 +138,31 @@ class VegasFlowPlus(VegasFlow):
         return int(self._offsets[self._n_cubes].item())
 
     # ------------------------------------------------------------ sampling
-    def _digest_random_generation(self, n_events, n_ev=None):
-        """Points in the hypercubes (vflowplus.py:37-80, 165-171): x, w
-        (with xjac = 1/n_cubes and limits), ind (uint8), segm (cube)."""
+    @native_hook
+    def _digest_random_generation(self, rnds, n_ev=None):
+        """Points in the hypercubes (vflowplus.py:37-80, 165-171): x, w,
+        ind (uint8), segm (cube of each event).
+
+        ``rnds`` an event count (a whole iteration): drawn and mapped in one
+        kernel, w including xjac = 1/n_cubes and the limits. ``rnds`` a tensor
+        of uniforms (the reference's contract): those uniforms placed in the
+        cubes and mapped, w = prod(dx * 50) / n_ev without xjac or limits."""
         self._require_device()
         if n_ev is not None and n_ev is not self._n_ev:
             self.n_ev = n_ev
         total = self._current_events()
+        if torch.is_tensor(rnds):
+            u = rnds.to(self.device).to(torch.float64).contiguous()
+            if tuple(u.shape) != (total, self.n_dim):
+                raise ValueError(f"uniforms must have shape ({total}, {self.n_dim}), got {tuple(u.shape)}")
+            step = kernels.make_step(self.n_dim, total, precision="f64", train=True, xjac=1.0, inject=u)
+            x = torch.empty((total, self.n_dim), dtype=torch.float64, device=self.device)
+            w = torch.empty((total,), dtype=torch.float64, device=self.device)
+            ind = torch.empty((total, self.n_dim), dtype=torch.uint8, device=self.device)
+            seg = torch.empty((total,), dtype=torch.int32, device=self.device)
+            kernels.vplus_sample(step, self._strata(), self.divisions, x, w, ind, seg)
+            return x.to(self._dtype), w.to(self._dtype), ind, seg
+        n_events = int(rnds)
         if n_events != total:
             raise ValueError(f"VegasFlowPlus draws whole iterations ({total} events), got {n_events}")
         step = self._step(total, xjac=self._modified_jac)
@@ -173,14 +194,23 @@ class VegasFlowPlus(VegasFlow):
         if f.dtype != self._dtype:
             f = f.to(self._dtype)
         f = f.contiguous()
+        xjac = torch.as_tensor(xjac, device=self.device).to(self._dtype).reshape(-1).contiguous()
+        ind = ind.to(torch.uint8).contiguous()
+        segm = segm.to(torch.int32).contiguous()
+        custom_hist = self.train and not self._hooks_native(VegasFlowPlus, ("_importance_sampling_array_filling",))
         ress = torch.zeros(self._n_cubes, dtype=torch.float64, device=self.device)
         ress2 = torch.zeros(self._n_cubes, dtype=torch.float64, device=self.device)
         ws = self._workspace_for(kernels.partial_stride(self.n_dim, 1, self.train))
-        step = self._step(n, train=self.train, offset=0, xjac=self._modified_jac)
-        ng = kernels.vplus_digest(step, self._strata(), f, xjac, ind if self.train else None, segm, ress,
-                                  ress2, ws)
+        hist_in_kernel = self.train and not custom_hist
+        step = self._step(n, train=hist_in_kernel, offset=0, xjac=self._modified_jac)
+        ng = kernels.vplus_digest(step, self._strata(), f, xjac, ind if hist_in_kernel else None,
+                                  segm, ress, ress2, ws)
         arr_res2 = []
-        if self.train:
+        if custom_hist:
+            # a subclass's histogram hook receives (w f)^2 and the bins
+            # (vflowplus.py:218); the digest wrote per-cube sums only
+            arr_res2 = self._importance_sampling_array_filling(torch.square(xjac * f), ind)
+        elif self.train:
             arr_res2 = torch.empty((self.n_dim, BINS_MAX), dtype=torch.float64, device=self.device)
             kernels.vegas_finalize(self.n_dim, 1, True, False, n, ws, ng, hist_out=arr_res2, raw=True)
         fn_ev = self._n_ev.to(torch.float64)
@@ -239,9 +269,16 @@ class VegasFlowPlus(VegasFlow):
                           inject=inject)
         strata = self._strata()
         ng = kernels.vplus_fused(step, strata, self._fused, self.divisions, self._ress, self._ress2, ws)
+        own_refine = self._hooks_native(VegasFlowPlus, ("refine_grid",))
+        hist = None
+        if self.train and not own_refine:
+            hist = torch.empty((self.n_dim, BINS_MAX), dtype=torch.float64, device=self.device)
         out = kernels.vplus_finalize(self.n_dim, strata, self._n_ev, self._offsets, self._ress, self._ress2,
                                      self._adaptive, self.min_neval_hcube, self._init_calls, BETA, ws, ng,
-                                     self.train, self.train, self.divisions)
+                                     self.train, self.train and own_refine, self.divisions, hist_out=hist)
+        if hist is not None:
+            # a subclass's refine_grid receives the histogram (vflowplus.py:239-240)
+            self.refine_grid(hist)
         self._pending_events.append(self._last_out[2] if self._last_out is not None else self._n_events)
         self._last_out = out
         return out[0], out[1]
