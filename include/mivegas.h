@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MV_ABI_VERSION 2
+#define MV_ABI_VERSION 3
 #define MV_BINS 50             /* BINS_MAX, configflow.py:13 */
 #define MV_FUSED_MAX_DIM 16    /* largest n_dim the fused kernels take */
 #define MV_MAX_COMP 4          /* components of the built-in mixture family */
@@ -41,6 +41,17 @@ extern "C" {
 #define MV_MAX_BLOCKS 2048     /* workgroups of one producer launch */
 #define MV_ROWS 32             /* group rows of a partials workspace */
 #define MV_DIGEST_DIMS 16      /* dimensions per grid.y slice of the digest */
+
+/* Bits of the status word of the finalize calls (mv_finalize.status,
+ * mv_vplus_finalize). A non-finite w*f at any
+ * event (an integrand returning NaN or +-Inf) makes the iteration's sum of
+ * w*f or of (w*f)^2 non-finite, since NaN and Inf survive additions and
+ * squares, so the finalize kernel tests those sums once per iteration
+ * rather than every event. It then ORs MV_STATUS_NONFINITE into the word
+ * (the caller zeroes it) and leaves the grid unrefined, so the grid stays
+ * finite for the next iteration; res and sigma are reported as computed
+ * (NaN or Inf, as the reference's arithmetic gives them). */
+#define MV_STATUS_NONFINITE 1u
 
 typedef enum {
   MV_OK = 0,
@@ -140,6 +151,7 @@ typedef struct {
   double* history;
   int32_t history_len;
   int32_t reserved2;
+  uint32_t* status;      /* nullable: MV_STATUS_* bits are ORed in */
 } mv_finalize;
 
 /* ---- library ---------------------------------------------------------- */
@@ -241,13 +253,15 @@ int64_t mv_vplus_out_len(int32_t n_cubes);
  * hist_out (nullable) and, with refine, the grid is refined. arr_var_out
  * (nullable, n_cubes) receives ress2 * n_ev - ress^2. draw_dev, history,
  * history_len: the iteration counter of mv_finalize (nullable); the events
- * recorded are those of the iteration just finalized, offsets[n_cubes]. */
+ * recorded are those of the iteration just finalized, offsets[n_cubes].
+ * status (nullable): as mv_finalize.status (a non-finite res also leaves
+ * n_ev unchanged: the reallocation's denominator is then not positive). */
 int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t* n_ev, int32_t* offsets,
                       double* ress, double* ress2, int32_t adaptive, int32_t min_neval,
                       double init_calls, double beta, double* rows, int64_t row_stride,
                       int32_t n_rows, int32_t train, int32_t refine, double* grid, double* out,
                       double* arr_var_out, double* hist_out, uint32_t* draw_dev,
-                      double* history, int32_t history_len, void* stream);
+                      double* history, int32_t history_len, uint32_t* status, void* stream);
 
 /* ---- batched independent integrals ------------------------------------- */
 /* No counterpart in the reference (each VegasFlow instance there runs its

@@ -10,7 +10,7 @@ import threading
 
 from . import _build
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 BINS = 50
 FUSED_MAX_DIM = 16
 MAX_COMP = 4
@@ -33,6 +33,8 @@ ERR_UNSUPPORTED = -2
 ERR_LAUNCH = -3
 ERR_WORKSPACE = -4
 ERR_DEVICE = -5
+
+STATUS_NONFINITE = 1
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -101,6 +103,7 @@ class Finalize(ctypes.Structure):
         ("history", _vp),
         ("history_len", _i32),
         ("reserved2", _i32),
+        ("status", _vp),
     ]
 
 
@@ -152,7 +155,7 @@ SIGNATURES = {
     "mv_vplus_out_len": (_i64, [_i32]),
     "mv_vplus_finalize": (ctypes.c_int, [_i32, _P(Strata), _vp, _vp, _vp, _vp, _i32, _i32, _f64,
                                          _f64, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
-                                         _vp, _vp, _i32, _vp]),
+                                         _vp, _vp, _i32, _vp, _vp]),
     "mv_batch_fused": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "mv_batch_fused_pairs": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "mv_batch_pairs_blocks": (_i32, [_i32, _i32]),

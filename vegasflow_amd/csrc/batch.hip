@@ -229,7 +229,7 @@ __global__ __launch_bounds__(64) void batch_finalize_kernel(const mv_batch_item*
       row[0] = 0.0;
       row[1] = 0.0;
       const double err = (N * res2 - res * res) / (N - 1.0);  // vflow.py:437
-      const double sig = sqrt(fmax(err, 0.0));                // vflow.py:438
+      const double sig = sqrt_clamped(err);                   // vflow.py:438
       double* st = stats + 4 * (int64_t)item;
       const double wgt = 1.0 / (sig * sig);                   // monte_carlo.py:718
       st[0] = res;
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(64) void batch_finalize_kernel(const mv_batch_item*
     row[c] = 0.0;
   }
   __syncthreads();
-  if (it.train) refine_wave(hb, grids + it.grid_offset + (int64_t)dim * (kBins + 1));
+  if (it.train && wave_hist_finite(hb)) refine_wave(hb, grids + it.grid_offset + (int64_t)dim * (kBins + 1));
 }
 
 }  // namespace mv

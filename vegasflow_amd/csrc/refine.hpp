@@ -97,7 +97,28 @@ struct FzK {
   int n_dim, n_out, train, refine, raw;
   double n_events;
   CounterK counter;  // zero: no counter
+  uint32_t* status;  // nullable: MV_STATUS_* bits
 };
+
+// Non-finite iteration sums (an integrand that returned NaN or Inf at some
+// event: both survive the additions and squares of the sums) set
+// MV_STATUS_NONFINITE in the caller's status word.
+__device__ __forceinline__ void flag_nonfinite(uint32_t* status, double a, double b) {
+  if (status && !(isfinite(a) && isfinite(b))) atomicOr(status, MV_STATUS_NONFINITE);
+}
+
+// sqrt(max(err, 0)) with the reference's NaN behaviour (vflow.py:438:
+// tf.maximum passes a NaN through; fmax would turn it into 0).
+__device__ __forceinline__ double sqrt_clamped(double err) { return sqrt(err < 0.0 ? 0.0 : err); }
+
+// The 50 bins of a histogram are all finite (whole wave votes; lanes >= 50
+// vote finite). Refinement is skipped otherwise, so that an iteration with
+// non-finite sums leaves the grid as it was.
+__device__ __forceinline__ bool wave_hist_finite(const double* hb) {
+  const int lane = threadIdx.x & 63;
+  const bool bad = lane < kBins && !isfinite(hb[lane]);
+  return !__any(bad);
+}
 
 // Sum column c of the (at most MV_ROWS) group rows in order and zero it for
 // the next producer. All loads are issued before the first store so that
@@ -133,9 +154,10 @@ __device__ inline void finalize_sums_block(const FzK& z, double* __restrict__ P,
     double second = res2;
     if (!z.raw) {
       const double err = (N * res2 - res * res) / (N - 1.0);  // vflow.py:437
-      second = sqrt(fmax(err, 0.0));                          // vflow.py:438
+      second = sqrt_clamped(err);                             // vflow.py:438
     }
     out[z.n_out + t] = second;
+    flag_nonfinite(z.status, res, res2);
     if (t == 0) advance_counter(z.counter, res, second, N);
   }
 }
@@ -154,7 +176,7 @@ __device__ inline void hist_refine_block(const FzK& z, double* __restrict__ P, i
     if (hist_out) hist_out[j * kBins + t] = a;
   }
   __syncthreads();
-  if (z.refine && t < 64) refine_wave(hb, grid + j * (kBins + 1));
+  if (z.refine && t < 64 && wave_hist_finite(hb)) refine_wave(hb, grid + j * (kBins + 1));
 }
 
 }  // namespace mv

@@ -450,7 +450,7 @@ __global__ __launch_bounds__(64) void refine_kernel(const double* __restrict__ h
   const int j = blockIdx.x, t = threadIdx.x;
   if (t < kBins) hb[t] = hist[j * hstride + t];
   __syncthreads();
-  refine_wave(hb, grid + j * (kBins + 1));
+  if (wave_hist_finite(hb)) refine_wave(hb, grid + j * (kBins + 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -740,7 +740,7 @@ extern "C" int mv_vegas_finalize(const mv_finalize* fz, double* rows, int64_t ro
   if (hist_out && !fz->train) return fail(MV_ERR_ARG, "mv_vegas_finalize: no histogram without train");
   if (fz->history && fz->history_len < 0) return fail(MV_ERR_ARG, "mv_vegas_finalize: history_len < 0");
   FzK z{fz->n_dim, fz->n_out, fz->train, fz->refine, fz->raw, fz->n_events,
-        CounterK{fz->draw_dev, fz->history, fz->history_len}};
+        CounterK{fz->draw_dev, fz->history, fz->history_len}, fz->status};
   const int blocks = 1 + (fz->train ? fz->n_dim : 0);  // histogram columns are always consumed
   hipLaunchKernelGGL(finalize_kernel, dim3(blocks), dim3(kBlock), 0, as_stream(stream), z, rows,
                      row_stride, n_rows, grid, out, hist_out);
@@ -771,3 +771,4 @@ extern "C" int mv_consume_array_into_indices(const double* input, const int32_t*
   }
   return check_launch("mv_consume_array_into_indices");
 }
+

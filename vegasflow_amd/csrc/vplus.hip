@@ -245,7 +245,8 @@ __global__ __launch_bounds__(kBlock) void vplus_realloc(int n_cubes, int32_t* __
                                                         const int32_t* __restrict__ offsets,
                                                         const double* __restrict__ ress2, int adaptive,
                                                         int min_neval, double init_calls,
-                                                        double* __restrict__ out, CounterK counter) {
+                                                        double* __restrict__ out, CounterK counter,
+                                                        uint32_t* status) {
   __shared__ double tot[3];
   __shared__ double part[kWaves];
   const int nb = (n_cubes + kBlock - 1) / kBlock;
@@ -267,6 +268,7 @@ __global__ __launch_bounds__(kBlock) void vplus_realloc(int n_cubes, int32_t* __
     if (blockIdx.x == 0) {
       out[0] = tot[0];
       out[1] = sqrt(tot[1]);  // vflowplus.py:233
+      flag_nonfinite(status, out[0], tot[1]);
       // this iteration's events: offsets are rewritten only by vplus_offsets
       advance_counter(counter, out[0], out[1], (double)offsets[n_cubes]);
     }
@@ -411,7 +413,7 @@ extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t
                                  int32_t n_rows, int32_t train, int32_t refine, double* grid,
                                  double* out, double* arr_var_out, double* hist_out,
                                  uint32_t* draw_dev, double* history, int32_t history_len,
-                                 void* stream) {
+                                 uint32_t* status, void* stream) {
   int rc = validate_strata(strata, n_dim, "mv_vplus_finalize");
   if (rc) return rc;
   if (history && history_len < 0) return fail(MV_ERR_ARG, "mv_vplus_finalize: history_len < 0");
@@ -433,9 +435,9 @@ extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t
                      beta, out, arr_var_out);
   if ((rc = check_launch("mv_vplus_finalize"))) return rc;
   hipLaunchKernelGGL(vplus_realloc, dim3(nb), dim3(kBlock), 0, s, n_cubes, n_ev, offsets, ress2, adaptive,
-                     min_neval, init_calls, out, CounterK{draw_dev, history, history_len});
+                     min_neval, init_calls, out, CounterK{draw_dev, history, history_len}, status);
   if ((rc = check_launch("mv_vplus_finalize"))) return rc;
-  const FzK z{n_dim, 1, 1, refine, 0, 0.0};
+  const FzK z{n_dim, 1, 1, refine, 0, 0.0, CounterK{}, nullptr};
   hipLaunchKernelGGL(vplus_offsets, dim3(nb + (train ? n_dim : 0)), dim3(kBlock), 0, s, n_cubes, n_ev,
                      offsets, ress, ress2, out, z, rows, row_stride, n_rows, grid, hist_out);
   return check_launch("mv_vplus_finalize");

@@ -254,10 +254,11 @@ def vegas_digest(step, f, w, idx, ws, main_dim=0):
 
 
 def vegas_finalize(n_dim, n_out, train, refine, n_events, ws, n_rows, grid=None, hist_out=None,
-                   raw=False, counter=None):
+                   raw=False, counter=None, status=None):
     """Reduce group rows; returns a (2 * n_out,) float64 tensor [res, sigma]
     (or [res, sum tmp^2] with raw). ``counter``: an IterationCounter the
-    kernel records the result in and advances."""
+    kernel records the result in and advances. ``status``: an int32 device
+    word the kernel ORs MV_STATUS_NONFINITE into for non-finite sums."""
     out = torch.empty(2 * n_out, dtype=torch.float64, device=ws.device)
     _need_device(grid, "grid", torch.float64)
     _need_device(hist_out, "hist_out", torch.float64)
@@ -272,6 +273,9 @@ def vegas_finalize(n_dim, n_out, train, refine, n_events, ws, n_rows, grid=None,
         fz.draw_dev = counter.draw.data_ptr()
         fz.history = counter.history.data_ptr()
         fz.history_len = counter.capacity
+    if status is not None:
+        _need_device(status, "status", torch.int32)
+        fz.status = status.data_ptr()
     _abi.call("mv_vegas_finalize", ctypes.byref(fz), _ptr(ws.rows), ws.stride, int(n_rows),
               _ptr(grid), _ptr(out), _ptr(hist_out), _stream())
     return out
@@ -350,9 +354,11 @@ def vplus_digest(step, strata, f, w, idx, seg, ress, ress2, ws):
 
 
 def vplus_finalize(n_dim, strata, n_ev, offsets, ress, ress2, adaptive, min_neval, init_calls, beta,
-                   ws, n_rows, train, refine, grid, arr_var_out=None, hist_out=None, counter=None):
+                   ws, n_rows, train, refine, grid, arr_var_out=None, hist_out=None, counter=None,
+                   status=None):
     """Returns a (3,) float64 tensor: res, sigma, events of the next iteration.
     ``counter``: an IterationCounter, as in vegas_finalize."""
+    _need_device(status, "status", torch.int32)
     out = torch.empty(int(_abi.load().mv_vplus_out_len(int(strata.n_cubes))), dtype=torch.float64,
                       device=ws.device)
     _abi.call("mv_vplus_finalize", int(n_dim), ctypes.byref(strata), _ptr(n_ev), _ptr(offsets), _ptr(ress),
@@ -361,7 +367,7 @@ def vplus_finalize(n_dim, strata, n_ev, offsets, ress, ress2, adaptive, min_neva
               _ptr(grid), _ptr(out), _ptr(arr_var_out), _ptr(hist_out),
               None if counter is None else counter.draw.data_ptr(),
               None if counter is None else counter.history.data_ptr(),
-              0 if counter is None else counter.capacity, _stream())
+              0 if counter is None else counter.capacity, _ptr(status), _stream())
     return out[:3]
 
 

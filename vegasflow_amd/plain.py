@@ -31,7 +31,8 @@ class PlainFlow(MonteCarloFlow):
         ws = self._workspace_for(kernels.partial_stride(self.n_dim, n_out, False))
         step = self._step(n_events, train=False, offset=0)
         ng = kernels.vegas_digest(step, f, xjac, None, ws)
-        out = kernels.vegas_finalize(self.n_dim, n_out, False, False, n_events, ws, ng, raw=True)
+        out = kernels.vegas_finalize(self.n_dim, n_out, False, False, n_events, ws, ng, raw=True,
+                                     status=self._status_word())
         if n_out == 1:
             return out[0], out[1]
         return out[:n_out], out[n_out:]
@@ -60,7 +61,8 @@ class PlainFlow(MonteCarloFlow):
         ws = self._workspace_for(kernels.partial_stride(self.n_dim, 1, False))
         step = self._step(n, train=False, offset=0, draw_dev=counter.draw)
         ng = kernels.plain_fused(step, self._fused, ws)
-        kernels.vegas_finalize(self.n_dim, 1, False, False, n, ws, ng, counter=counter)
+        kernels.vegas_finalize(self.n_dim, 1, False, False, n, ws, ng, counter=counter,
+                               status=self._status_word())
 
     def _fused_iteration(self, inject=None):
         self._next_draw()
@@ -68,7 +70,7 @@ class PlainFlow(MonteCarloFlow):
         ws = self._workspace_for(kernels.partial_stride(self.n_dim, 1, False))
         step = self._step(n, train=False, offset=0, inject=inject)
         ng = kernels.plain_fused(step, self._fused, ws)
-        out = kernels.vegas_finalize(self.n_dim, 1, False, False, n, ws, ng)
+        out = kernels.vegas_finalize(self.n_dim, 1, False, False, n, ws, ng, status=self._status_word())
         return out[0], out[1]
 
 

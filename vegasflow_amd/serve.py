@@ -10,6 +10,13 @@ host reads the per-slot iteration counters and running weighted averages,
 retires the requests that reached their target (after ``min_iter``
 iterations) or their cap, and admits waiting requests into the freed slots.
 
+Failure handling: ``cancel`` withdraws a waiting or running request; a
+request with a ``timeout`` (seconds from its arrival) is retired at the
+first tick past it, or dropped unadmitted; a request whose integral is not
+finite (its integrand gave NaN or Inf) is retired at once. Each ends with
+``status`` "cancelled", "timeout" or "nonfinite" and its last result;
+converged requests end "converged", those at their iteration cap "max_iter".
+
 A request's draws depend only on its seed, its iteration and its event
 index, so its result is the same alone and in the mix up to summation order
 (``run_alone`` reproduces it with VegasFlow).
@@ -43,6 +50,7 @@ class Request:
     seed: int = 0
     arrival: float = 0.0
     rid: int = -1
+    timeout: float = None  # seconds after arrival (None: no limit)
     # filled by the server
     admitted: float = None
     done: float = None
@@ -50,6 +58,7 @@ class Request:
     error: float = None
     iterations: int = 0
     slot: int = -1
+    status: str = "waiting"  # running, converged, max_iter, cancelled, timeout, nonfinite
 
     @property
     def latency(self):
@@ -127,8 +136,35 @@ class Server:
         self.stats[4 * s:4 * s + 4] = 0.0
         req.slot = s
         req.admitted = now
+        req.status = "running"
         self.live[s] = req
         self._launch = None
+
+    def _retire(self, req, now, status):
+        """End ``req`` (running or waiting) with ``status``; a slot it held
+        is free for the next admission (the last tick's finalize consumed
+        its row, so the slot is clean)."""
+        req.done = now
+        req.status = status
+        if req.slot in self.live and self.live[req.slot] is req:
+            del self.live[req.slot]
+            self.free.append(req.slot)
+            self._launch = None
+        self.finished.append(req)
+
+    def cancel(self, req, now=None):
+        """Withdraw a waiting or running request (between ticks); returns
+        False if it had already finished."""
+        now = time.perf_counter() if now is None else now
+        if req.status == "waiting":
+            at = next((i for i, q in enumerate(self.queue) if q is req), None)
+            if at is None:
+                return False
+            del self.queue[at]
+        elif not (req.status == "running" and self.live.get(req.slot) is req):
+            return False
+        self._retire(req, now, "cancelled")
+        return True
 
     def _tiles_of(self, s, req, ept):
         key = (s, req.n_events, ept)
@@ -216,19 +252,27 @@ class Server:
             req.iterations = k
             req.result, req.error = float(est), float(err)
             converged = k >= self.min_iter and np.isfinite(err) and err <= req.target * abs(est)
-            if converged or k >= req.max_iter:
-                req.done = now
-                retired.append(req)
-                del self.live[s]
-                self.free.append(s)
-        if retired:
-            self._launch = None
-        self.finished.extend(retired)
+            if not (np.isfinite(st[0]) and np.isfinite(st[1])):
+                status = "nonfinite"  # this iteration's integral: NaN or Inf
+            elif converged:
+                status = "converged"
+            elif k >= req.max_iter:
+                status = "max_iter"
+            elif req.timeout is not None and now - req.arrival >= req.timeout:
+                status = "timeout"
+            else:
+                continue
+            self._retire(req, now, status)
+            retired.append(req)
         return retired
 
     def admit_waiting(self, now):
         while self.queue and self.free and self.queue[0].arrival <= now:
-            self._admit(self.queue.popleft(), now)
+            req = self.queue.popleft()
+            if req.timeout is not None and now - req.arrival >= req.timeout:
+                self._retire(req, now, "timeout")  # expired while waiting
+                continue
+            self._admit(req, now)
 
     # -------------------------------------------------------------- loops
     def warm_up(self, requests):
@@ -323,7 +367,7 @@ def latency_report(finished):
             "p50_ms": float(np.percentile(cl, 50) * 1e3) if cl.size else None,
             "p99_ms": float(np.percentile(cl, 99) * 1e3) if cl.size else None,
             "mean_iterations": float(np.mean([r.iterations for r in rs])),
-            "met_target": float(np.mean([r.error <= r.target * abs(r.result) for r in rs])),
+            "met_target": float(np.mean([_met(r) for r in rs])),
         }
     return {
         "requests": int(lat.size),
@@ -331,6 +375,11 @@ def latency_report(finished):
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "max_ms": float(lat.max() * 1e3),
         "mean_iterations": float(np.mean([r.iterations for r in finished])),
-        "met_target": float(np.mean([r.error <= r.target * abs(r.result) for r in finished])),
+        "met_target": float(np.mean([_met(r) for r in finished])),
+        "status": dict(collections.Counter(r.status for r in finished)),
         "by_class": by_class,
     }
+
+
+def _met(req):
+    return req.result is not None and req.error is not None and bool(req.error <= req.target * abs(req.result))

@@ -212,7 +212,8 @@ class VegasFlowPlus(VegasFlow):
             arr_res2 = self._importance_sampling_array_filling(torch.square(xjac * f), ind)
         elif self.train:
             arr_res2 = torch.empty((self.n_dim, BINS_MAX), dtype=torch.float64, device=self.device)
-            kernels.vegas_finalize(self.n_dim, 1, True, False, n, ws, ng, hist_out=arr_res2, raw=True)
+            kernels.vegas_finalize(self.n_dim, 1, True, False, n, ws, ng, hist_out=arr_res2, raw=True,
+                                   status=self._status_word())
         fn_ev = self._n_ev.to(torch.float64)
         arr_var = ress2 * fn_ev - torch.square(ress)
         return ress, arr_var, arr_res2
@@ -254,7 +255,8 @@ class VegasFlowPlus(VegasFlow):
         ng = kernels.vplus_fused(step, strata, self._fused, self.divisions, self._ress, self._ress2, ws)
         return kernels.vplus_finalize(self.n_dim, strata, self._n_ev, self._offsets, self._ress, self._ress2,
                                       self._adaptive, self.min_neval_hcube, self._init_calls, BETA, ws, ng,
-                                      self.train, self.train, self.divisions, counter=counter)
+                                      self.train, self.train, self.divisions, counter=counter,
+                                      status=self._status_word())
 
     def _replayed_events(self, events):
         # events of each replayed iteration, and the next iteration's count
@@ -275,7 +277,8 @@ class VegasFlowPlus(VegasFlow):
             hist = torch.empty((self.n_dim, BINS_MAX), dtype=torch.float64, device=self.device)
         out = kernels.vplus_finalize(self.n_dim, strata, self._n_ev, self._offsets, self._ress, self._ress2,
                                      self._adaptive, self.min_neval_hcube, self._init_calls, BETA, ws, ng,
-                                     self.train, self.train and own_refine, self.divisions, hist_out=hist)
+                                     self.train, self.train and own_refine, self.divisions, hist_out=hist,
+                                     status=self._status_word())
         if hist is not None:
             # a subclass's refine_grid receives the histogram (vflowplus.py:239-240)
             self.refine_grid(hist)
