@@ -41,6 +41,8 @@ TRAFFIC_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_pmc.md"
 # every 4 cycles, 4 SIMDs per CU, 2.4 GHz engine clock (MI355X datasheet)
 VALU_INSTS = {"mixed": 2.535e7, "f64": 5.093e7}
 CLOCK_HZ = 2.4e9
+# host cores one GPU's share of a GPU box holds (worker pools are sized to it)
+CPU_SHARE = 16
 
 
 def flops_per_event(d, train=True):
@@ -253,6 +255,16 @@ def bench_serving(seconds, dtype, rate=200.0):
 
 
 def _cpu_model():
+    """lscpu's model name (the same line /proc/cpuinfo holds)."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
@@ -279,7 +291,10 @@ def cpu_baseline(numpy_seconds=6.0):
     from oracle import torch_cpu as tc
     from oracle import vegas as ora
 
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    # the host cores this process may run on, capped at the CPU share a GPU
+    # box gives one GPU (16 cores; the box's process guard sizes pools to it)
+    affinity = len(os.sched_getaffinity(0))
+    threads = max(1, min(CPU_SHARE, affinity))
     torch.set_num_threads(threads)
     gen = torch.Generator().manual_seed(SEED)
     div = torch.from_numpy(np.ascontiguousarray(ora.initial_grid(D)))
@@ -308,6 +323,8 @@ def cpu_baseline(numpy_seconds=6.0):
         "value": done / t_torch,
         "unit": "events/s",
         "cores": threads,
+        "cores_in_affinity": affinity,
+        "cpu_share_per_gpu": CPU_SHARE,
         "kind": "port",
         "sample": f"one training iteration of {done} events (10 steps of 1e6), d=8 Gaussian, FP64, "
                   f"torch-CPU op-for-op restatement of the reference graph, {threads} threads, "
@@ -352,7 +369,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-protocol", action="store_true", help="skip the 5+5 iteration error check")
     ap.add_argument("--no-extra", action="store_true", help="skip the VEGAS+, batched and serving runs")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the Poisson stream")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the Poisson stream (60 s: the serving configuration)")
     ap.add_argument("--only", choices=["unfused", "plain", "vplus", "batch", "serve"],
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
