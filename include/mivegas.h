@@ -43,7 +43,7 @@ extern "C" {
 #define MV_DIGEST_DIMS 16      /* dimensions per grid.y slice of the digest */
 
 /* Bits of the status word of the finalize calls (mv_finalize.status,
- * mv_vplus_finalize). A non-finite w*f at any
+ * mv_vplus_finalize, mv_vegas_iteration_static). A non-finite w*f at any
  * event (an integrand returning NaN or +-Inf) makes the iteration's sum of
  * w*f or of (w*f)^2 non-finite, since NaN and Inf survive additions and
  * squares, so the finalize kernel tests those sums once per iteration
@@ -262,6 +262,24 @@ int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t* n_ev, int
                       int32_t n_rows, int32_t train, int32_t refine, double* grid, double* out,
                       double* arr_var_out, double* hist_out, uint32_t* draw_dev,
                       double* history, int32_t history_len, uint32_t* status, void* stream);
+
+/* One VegasFlow iteration (mv_vegas_fused then mv_vegas_finalize with
+ * n_out = 1, N = step->n_events) that allocates nothing: no caller
+ * workspace, and no hipMalloc / hipFree or allocating library call. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid it). The partial rows live in a static device array
+ * of MV_ROWS x (2 + MV_FUSED_MAX_DIM * MV_BINS) doubles (205 KB) that each
+ * GPU the library runs on holds until the process ends, so n_dim is at most
+ * MV_FUSED_MAX_DIM (larger is rejected with MV_ERR_UNSUPPORTED). out (2
+ * doubles) = {res, sigma}; hist_out (nullable, (n_dim, MV_BINS)) and
+ * refine (grid refined in place) need step->train; status (nullable) as
+ * mv_finalize.status. Every call on a GPU
+ * shares that GPU's array and leaves it zeroed: calls that overlap on one
+ * GPU (different streams, threads) race and must not be issued. */
+int mv_vegas_iteration_static(const mv_step* step, const mv_integrand* fn, double* grid,
+                              int32_t refine, double* out, double* hist_out, uint32_t* status,
+                              void* stream);
 
 /* ---- batched independent integrals ------------------------------------- */
 /* No counterpart in the reference (each VegasFlow instance there runs its

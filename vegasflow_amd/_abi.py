@@ -156,6 +156,7 @@ SIGNATURES = {
     "mv_vplus_finalize": (ctypes.c_int, [_i32, _P(Strata), _vp, _vp, _vp, _vp, _i32, _i32, _f64,
                                          _f64, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _i32, _vp, _vp]),
+    "mv_vegas_iteration_static": (ctypes.c_int, [_P(Step), _P(Integrand), _vp, _i32, _vp, _vp, _vp, _vp]),
     "mv_batch_fused": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "mv_batch_fused_pairs": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "mv_batch_pairs_blocks": (_i32, [_i32, _i32]),

@@ -772,3 +772,83 @@ extern "C" int mv_consume_array_into_indices(const double* input, const int32_t*
   return check_launch("mv_consume_array_into_indices");
 }
 
+// ---------------------------------------------------------------------------
+// One iteration without any allocation (mv_vegas_iteration_static)
+// ---------------------------------------------------------------------------
+// The partial rows of mv_vegas_fused + mv_vegas_finalize live in a
+// file-scope device array instead of a caller workspace: MV_ROWS rows of
+// kStaticStride doubles (2 sums + MV_FUSED_MAX_DIM * MV_BINS histogram
+// bins), 205 KB held by every GPU the library runs on until the process
+// ends. Host code never passes the array's name to a kernel: that is a
+// host-side stand-in, not a device address. The address of each GPU's copy
+// comes from hipGetSymbolAddress, once per device with that device current,
+// and is kept per device.
+namespace {
+
+constexpr int64_t kStaticStride = 2 + (int64_t)MV_FUSED_MAX_DIM * MV_BINS;
+constexpr int64_t kStaticDoubles = (int64_t)MV_ROWS * kStaticStride;
+constexpr int kStaticDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[kStaticDoubles];
+
+namespace {
+
+std::mutex g_static_mu;
+double* g_static_rows[kStaticDevices] = {};
+
+// This device's copy of ``scratch``. The first call on a device looks up
+// its address and zeroes it on ``st`` (a fill kernel, ordered before the
+// caller's work); every finalize zeroes the rows it consumes, so the rows
+// are zero again between calls.
+int static_rows(hipStream_t st, double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(MV_ERR_DEVICE, "mv_vegas_iteration_static: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kStaticDevices)
+    return fail(MV_ERR_DEVICE, "mv_vegas_iteration_static: device %d beyond %d", dev, kStaticDevices);
+  std::lock_guard<std::mutex> g(g_static_mu);
+  if (!g_static_rows[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MV_ERR_DEVICE, "mv_vegas_iteration_static: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    double* rows = static_cast<double*>(p);
+    hipLaunchKernelGGL(fill_kernel, dim3((kStaticDoubles + kBlock - 1) / kBlock), dim3(kBlock), 0, st, rows,
+                       kStaticDoubles, 0.0);
+    int rc = check_launch("mv_vegas_iteration_static");
+    if (rc) return rc;
+    g_static_rows[dev] = rows;
+  }
+  *out = g_static_rows[dev];
+  return MV_OK;
+}
+
+}  // namespace
+
+extern "C" int mv_vegas_iteration_static(const mv_step* step, const mv_integrand* fn, double* grid,
+                                         int32_t refine, double* out, double* hist_out, uint32_t* status,
+                                         void* stream) {
+  int rc = validate_step(step, "mv_vegas_iteration_static", MV_FUSED_MAX_DIM);
+  if (rc) return rc;
+  if ((rc = validate_fn(fn, "mv_vegas_iteration_static"))) return rc;
+  if (step->n_events < 1) return fail(MV_ERR_ARG, "mv_vegas_iteration_static: n_events must be >= 1");
+  if (!grid || !out) return fail(MV_ERR_ARG, "mv_vegas_iteration_static: grid and out are required");
+  if ((refine || hist_out) && !step->train)
+    return fail(MV_ERR_ARG, "mv_vegas_iteration_static: refine and hist_out need a training step");
+  hipStream_t st = as_stream(stream);
+  double* rows = nullptr;
+  if ((rc = static_rows(st, &rows))) return rc;
+  const mv_partials parts{rows, kStaticStride};
+  const int n_rows = mv_vegas_fused(step, fn, grid, &parts, stream);
+  if (n_rows < 0) return n_rows;
+  mv_finalize fz{};
+  fz.n_dim = step->n_dim;
+  fz.n_out = 1;
+  fz.train = step->train != 0;
+  fz.refine = refine != 0;
+  fz.n_events = (double)step->n_events;
+  fz.status = status;
+  return mv_vegas_finalize(&fz, rows, kStaticStride, n_rows, grid, out, hist_out, stream);
+}

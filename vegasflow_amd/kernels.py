@@ -209,6 +209,22 @@ def vegas_fused(step, fn_struct, grid, ws):
                      ws.partials(), _stream())
 
 
+def vegas_iteration_static(step, fn_struct, grid, refine, out, hist_out=None, status=None):
+    """mv_vegas_iteration_static: a whole iteration with the library's static
+    partial rows (no workspace); ``out`` (2,) float64 receives {res, sigma}."""
+    _need_device(grid, "grid", torch.float64)
+    _need_device(out, "out", torch.float64)
+    _need_device(hist_out, "hist_out", torch.float64)
+    if out.numel() < 2:
+        raise ValueError("out needs 2 entries")
+    if hist_out is not None and hist_out.numel() < step.n_dim * BINS:
+        raise ValueError("hist_out needs n_dim * 50 entries")
+    _need_device(status, "status", torch.int32)
+    _abi.call("mv_vegas_iteration_static", ctypes.byref(step), ctypes.byref(fn_struct), _ptr(grid),
+              int(bool(refine)), _ptr(out), _ptr(hist_out), _ptr(status), _stream())
+    return out
+
+
 def plain_fused(step, fn_struct, ws):
     return _abi.call("mv_plain_fused", ctypes.byref(step), ctypes.byref(fn_struct), ws.partials(),
                      _stream())
