@@ -159,7 +159,7 @@ int mv_abi_version(void);
 const char* mv_last_error(void);
 /* sizeof of the ABI structs, for binding-layout checks:
  * 0 mv_integrand, 1 mv_step, 2 mv_partials, 3 mv_strata, 4 mv_finalize,
- * 5 mv_batch_item, 6 mv_batch_tile. */
+ * 5 mv_batch_item, 6 mv_batch_tile, 7 mv_batch_vplus_item. */
 int64_t mv_struct_size(int32_t which);
 /* Doubles per partial row: 2 * n_out + (train ? n_dim * MV_BINS : 0). */
 int64_t mv_partial_stride(int32_t n_dim, int32_t n_out, int32_t train);
@@ -356,6 +356,63 @@ int32_t mv_batch_pairs_blocks_f64(int32_t family, int32_t max_dim);
 int mv_batch_finalize(const mv_batch_item* items, const int32_t* jobs, int32_t n_jobs,
                       double* grids, double* rows, int32_t* draws, double* stats,
                       double* history, int32_t max_iter, void* stream);
+
+/* ---- batched VegasFlowPlus integrals ----------------------------------- */
+/* No counterpart in the reference (batching is an addition): many
+ * VegasFlowPlus integrals (vflowplus.py:88-247) of n_dim <= MV_FUSED_MAX_DIM
+ * advance one iteration each in one persistent producer launch and three
+ * iteration-end launches, with their grids, event allocations, per-cube sums
+ * and iteration counters resident on the device. Item i draws exactly what
+ * VegasFlowPlus(n_dim_i, init_calls_i, adaptive = adaptive_i, seed = seed_i)
+ * would at iteration draws[i] (event index within the iteration as the
+ * Philox counter), so its results equal a lone run up to summation order. */
+typedef struct {
+  int32_t n_dim;
+  int32_t train;         /* refine this item's grid */
+  int32_t n_strat;       /* stratification of vflowplus.py:113-137 */
+  int32_t n_cubes;       /* n_strat^n_dim */
+  int32_t adaptive;      /* reallocate events (vflowplus.py:153-163) */
+  int32_t min_neval;
+  double init_calls;     /* events requested: N0 of the reallocation */
+  uint64_t seed;
+  int64_t grid_offset;   /* doubles from ``grids`` to its (n_dim, 51) grid */
+  int64_t row_offset;    /* doubles from ``rows`` to its 2 + 50 n_dim row */
+  int64_t cube_offset;   /* entries from n_ev, ress, ress2 (n_cubes each) */
+  int64_t off_offset;    /* entries from offsets (n_cubes + 1, exclusive
+                          * prefix sum of n_ev; offsets[n_cubes] = events
+                          * of the iteration) */
+  int64_t out_offset;    /* doubles from ``outs``: mv_vplus_out_len(n_cubes),
+                          * {res, sigma, events of the next iteration,
+                          * scratch} after each iteration end */
+  mv_integrand fn;
+} mv_batch_vplus_item;
+
+/* One iteration of the items' events listed in ``tiles`` (mv_batch_tile over
+ * event indices of an item's iteration; events at or beyond the item's
+ * offsets[n_cubes] are skipped, so tiles may cover an upper bound of the
+ * events, e.g. n_cubes * min_neval + init_calls / 2 + 1 when adaptive):
+ * draws in the events' hypercubes, grid map, integrand, per-cube sums into
+ * ress / ress2 and the (w f)^2 bin histogram into each item's row (columns
+ * 2 ..). Every tiled item has integrand family ``family``; max_dim is the
+ * largest n_dim among them; ``blocks`` persistent 256-thread workgroups. */
+int mv_batch_vplus_fused(const mv_batch_vplus_item* items, const mv_batch_tile* tiles, int32_t n_tiles,
+                         int32_t precision, int32_t family, int32_t max_dim, const int32_t* draws,
+                         const double* grids, const int32_t* n_ev, const int32_t* offsets, double* ress,
+                         double* ress2, double* rows, int32_t blocks, void* stream);
+
+/* Iteration end of the items (vflowplus.py:222-242, 153-163 and vflow.py:
+ * 349-362 per item, as mv_vplus_finalize): ``chunks`` lists device int32
+ * pairs {item, chunk} with chunk = 0 .. ceil(n_cubes / 256) - 1 for every
+ * item, ``dims`` pairs {item, dim} for the histogram columns of every item
+ * (refined when the item trains). Per item: outs = {res, sigma, events of
+ * the next iteration}, n_ev and offsets reallocated when adaptive, ress and
+ * ress2 and the row zeroed, stats[4 i ..] = {integral, sigma, sum 1/sigma^2,
+ * sum integral/sigma^2} (the running weighted average of monte_carlo.py:
+ * 713-732), draws[i] advanced. */
+int mv_batch_vplus_finalize(const mv_batch_vplus_item* items, const int32_t* chunks, int32_t n_chunks,
+                            const int32_t* dims, int32_t n_dims, double beta, double* grids, int32_t* n_ev,
+                            int32_t* offsets, double* ress, double* ress2, double* rows, double* outs,
+                            int32_t* draws, double* stats, void* stream);
 
 #ifdef __cplusplus
 }

@@ -89,7 +89,7 @@ def _req(fn, d, n, seed, **kw):
 
 
 def test_server_cancel_timeout_and_nonfinite():
-    srv = serve.Server(memory_budget=1 << 20)
+    srv = serve.Server(memory_budget=16 * serve.slot_bytes())
     srv.min_iter = 1
     keep = _req(vf.Gaussian(0.2, 0.5), 3, 20_000, seed=11)
     running = _req(vf.Gaussian(0.2, 0.5), 4, 20_000, seed=12)

@@ -1,11 +1,13 @@
 """Serving loop: continuous batching, admission by memory, retirement at the
 error target; each result equals the request run alone (same draws) up to
-summation order."""
+summation order, VegasFlow and VegasFlowPlus requests alike."""
 import numpy as np
 import pytest
 import torch
 
 import vegasflow_amd as vf
+from oracle import vegas as ora
+from tests.parity import RTOL, assert_sigma, kappa_vplus
 from vegasflow_amd.serve import Request, Server, latency_report, poisson_requests, run_alone, slot_bytes
 
 pytestmark = pytest.mark.gpu
@@ -16,14 +18,15 @@ def mixed_requests(n, seed=0):
     out = []
     for i in range(n):
         kind = i % 3
+        alg = "vegas"
         if kind == 0:
-            fn, d = vf.GaussianMix(), 4
+            fn, d, alg = vf.GaussianMix(), 4, "vegas+"
         elif kind == 1:
             fn, d = vf.BreitWignerPower(), 6
         else:
             fn, d = vf.Gaussian(0.2, 0.4), 5
         out.append(Request(fn, d, int(rng.integers(20_000, 120_000)), target=float(rng.choice([1e-2, 3e-3])),
-                           max_iter=12, seed=500 + i, arrival=0.002 * i, rid=i))
+                           max_iter=12, seed=500 + i, arrival=0.002 * i, rid=i, algorithm=alg))
     return out
 
 
@@ -70,3 +73,60 @@ def test_loose_targets_retire_early_and_report_latency():
     assert rep["p99_ms"] >= rep["p50_ms"] > 0
     assert all(r.iterations < 20 for r in done)
     assert rep["met_target"] == 1.0
+
+
+def mix_np(x):
+    return ora.gaussian_mix(x, (0.5, 0.5), (0.05, 0.1), (0.33, 0.67))
+
+
+@pytest.mark.parametrize("d,n,adaptive", [(4, 100_000, True), (3, 30_000, False), (2, 50_000, True)])
+def test_vplus_slot_first_iteration_matches_oracle(d, n, adaptive):
+    """One tick of a VegasFlowPlus slot against the oracle's VEGAS+ iteration
+    on the same Philox draws (vflowplus.py:37-80, 187-242): integral and
+    sigma at the 1e-12 bar, refined grid, and the reallocated events of the
+    next iteration; a VegasFlow request shares the ticks."""
+    server = Server(memory_budget=8 * slot_bytes())
+    server.min_iter = 99
+    req = Request(vf.GaussianMix(), d, n, target=1e-12, max_iter=3, seed=77 + d, algorithm="vegas+",
+                  adaptive=adaptive)
+    other = Request(vf.Gaussian(0.2, 0.4), 5, 40_000, target=1e-12, max_iter=3, seed=5)
+    server.submit(req)
+    server.submit(other)
+    server.admit_waiting(0.0)
+    server.tick()
+    st = ora.vplus_setup(d, n, adaptive)
+    total = int(st["n_ev"].sum())
+    u = ora.uniform_f64(req.seed, 0, total, d)
+    grid0 = ora.initial_grid(d)
+    res, sig, new_div, new_n_ev, o_var, _ = ora.vplus_iteration(grid0, st, u, mix_np)
+    s = req.slot
+    stats = server.stats[4 * s:4 * s + 4].cpu().numpy()
+    assert abs(stats[0] - res) <= RTOL * abs(res)
+    x, w, ind, seg = ora.generate_samples_in_hypercubes(u, st["n_strat"], st["n_ev"],
+                                                        ora.hypercubes(st["n_strat"], d), grid0)
+    x, w = ora.apply_xjac_and_limits(x, w, 1.0 / st["n_cubes"])
+    o_ress, _, _ = ora.vplus_event(x, w, ind, seg, st["n_ev"], mix_np(x), False)
+    assert_sigma(stats[1], sig, kappa_vplus(o_ress, o_var, st["n_ev"], sig))
+    grid = server.grids[s * 816:s * 816 + d * 51].view(d, 51).cpu().numpy()
+    np.testing.assert_allclose(grid, new_div, rtol=RTOL, atol=1e-15)
+    n_ev = server.n_ev[s * 10_000:s * 10_000 + st["n_cubes"]].cpu().numpy()
+    diff = np.abs(n_ev.astype(np.int64) - new_n_ev.astype(np.int64))
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    assert req._vp["events"] == int(n_ev.sum())
+    while server.live:
+        server.tick()
+    assert req.iterations == other.iterations == 3
+
+
+def test_vplus_requests_converge_where_the_grid_alone_does_not():
+    """The two-Gaussian mixture has its peaks on the diagonal, which a
+    separable grid cannot follow; stratification with adaptive allocation
+    reaches a 1e-3 target that plain VEGAS misses in the same budget."""
+    server = Server(memory_budget=8 * slot_bytes())
+    vp = Request(vf.GaussianMix(), 4, 200_000, target=1e-3, max_iter=20, seed=3, algorithm="vegas+")
+    vg = Request(vf.GaussianMix(), 4, 200_000, target=1e-3, max_iter=20, seed=3, algorithm="vegas")
+    done = server.run_stream([vp, vg], warm_up=False)
+    assert len(done) == 2
+    assert vp.status == "converged" and vp.iterations < vg.iterations
+    exact = vf.GaussianMix().exact(4)
+    assert abs(vp.result - exact) < 4 * vp.error

@@ -127,7 +127,26 @@ class BatchTile(ctypes.Structure):
     ]
 
 
-STRUCTS = [Integrand, Step, Partials, Strata, Finalize, BatchItem, BatchTile]
+class BatchVplusItem(ctypes.Structure):
+    _fields_ = [
+        ("n_dim", _i32),
+        ("train", _i32),
+        ("n_strat", _i32),
+        ("n_cubes", _i32),
+        ("adaptive", _i32),
+        ("min_neval", _i32),
+        ("init_calls", _f64),
+        ("seed", _u64),
+        ("grid_offset", _i64),
+        ("row_offset", _i64),
+        ("cube_offset", _i64),
+        ("off_offset", _i64),
+        ("out_offset", _i64),
+        ("fn", Integrand),
+    ]
+
+
+STRUCTS = [Integrand, Step, Partials, Strata, Finalize, BatchItem, BatchTile, BatchVplusItem]
 
 _P = ctypes.POINTER
 # name -> (restype, argtypes); the names are exactly those of include/mivegas.h
@@ -163,6 +182,10 @@ SIGNATURES = {
     "mv_batch_fused_pairs_f64": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "mv_batch_pairs_blocks_f64": (_i32, [_i32, _i32]),
     "mv_batch_finalize": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "mv_batch_vplus_fused": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp,
+                                            _vp, _i32, _vp]),
+    "mv_batch_vplus_finalize": (ctypes.c_int, [_vp, _vp, _i32, _vp, _i32, _f64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                               _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

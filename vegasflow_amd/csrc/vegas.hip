@@ -508,6 +508,7 @@ extern "C" int64_t mv_struct_size(int32_t which) {
     case 4: return (int64_t)sizeof(mv_finalize);
     case 5: return (int64_t)sizeof(mv_batch_item);
     case 6: return (int64_t)sizeof(mv_batch_tile);
+    case 7: return (int64_t)sizeof(mv_batch_vplus_item);
     default: return -1;
   }
 }

@@ -188,13 +188,15 @@ __device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* tota
 // ress2 * n_ev - ress^2 (to arr_var_out) and, when adaptive, the damped
 // variance max(var, 0)^(beta/2), left in ress2 for part 2; per workgroup the
 // partial sums of ress, max(var, 0) / (n_ev - 1) and the damped variances.
-__global__ __launch_bounds__(kBlock) void vplus_cube_stats(int n_cubes, const int32_t* __restrict__ n_ev,
-                                                           const double* __restrict__ ress,
-                                                           double* __restrict__ ress2, int adaptive,
-                                                           double beta, double* __restrict__ out,
-                                                           double* __restrict__ arr_var_out) {
+// The bodies of the three kernels take the workgroup's chunk ``b`` of cubes
+// explicitly, so that the batched iteration end (batch_vplus.hip) runs them
+// for many integrals in one launch each.
+__device__ inline void cube_stats_body(int b, int n_cubes, const int32_t* __restrict__ n_ev,
+                                       const double* __restrict__ ress, double* __restrict__ ress2,
+                                       int adaptive, double beta, double* __restrict__ out,
+                                       double* __restrict__ arr_var_out) {
   __shared__ double part[kWaves];
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int c = b * kBlock + threadIdx.x;
   double rs = 0.0, s2 = 0.0, dl = 0.0;
   if (c < n_cubes) {
     const double fn = (double)n_ev[c];
@@ -212,11 +214,19 @@ __global__ __launch_bounds__(kBlock) void vplus_cube_stats(int n_cubes, const in
   s2 = block_sum(s2, part);
   dl = block_sum(dl, part);
   if (threadIdx.x == 0) {
-    double* o = out + 3 + 3 * (int64_t)blockIdx.x;
+    double* o = out + 3 + 3 * (int64_t)b;
     o[0] = rs;
     o[1] = s2;
     o[2] = dl;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void vplus_cube_stats(int n_cubes, const int32_t* __restrict__ n_ev,
+                                                           const double* __restrict__ ress,
+                                                           double* __restrict__ ress2, int adaptive,
+                                                           double beta, double* __restrict__ out,
+                                                           double* __restrict__ arr_var_out) {
+  cube_stats_body(blockIdx.x, n_cubes, n_ev, ress, ress2, adaptive, beta, out, arr_var_out);
 }
 
 // The partial sums of part 1 over all workgroups, in order (every workgroup
@@ -241,18 +251,18 @@ __device__ __forceinline__ void cube_totals(const double* __restrict__ out, int 
 // Part 2 (vflowplus.py:158-162): n_ev = int(max(min_neval, damped *
 // init_calls / 2 / sum damped)) when adaptive (and the sum is positive), the
 // events per workgroup, and out[0..1] = {res, sigma} (vflowplus.py:231-233).
-__global__ __launch_bounds__(kBlock) void vplus_realloc(int n_cubes, int32_t* __restrict__ n_ev,
-                                                        const int32_t* __restrict__ offsets,
-                                                        const double* __restrict__ ress2, int adaptive,
-                                                        int min_neval, double init_calls,
-                                                        double* __restrict__ out, CounterK counter,
-                                                        uint32_t* status) {
+// Chunk 0's thread 0 writes out[0..1] and returns true (the caller may then
+// read them in that thread).
+__device__ inline bool realloc_body(int b, int n_cubes, int32_t* __restrict__ n_ev,
+                                    const int32_t* __restrict__ offsets, const double* __restrict__ ress2,
+                                    int adaptive, int min_neval, double init_calls, double* __restrict__ out,
+                                    CounterK counter, uint32_t* status) {
   __shared__ double tot[3];
   __shared__ double part[kWaves];
   const int nb = (n_cubes + kBlock - 1) / kBlock;
   cube_totals(out, nb, tot);
   const double dsum = tot[2];
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int c = b * kBlock + threadIdx.x;
   double ne = 0.0;
   if (c < n_cubes) {
     int v = n_ev[c];
@@ -264,21 +274,58 @@ __global__ __launch_bounds__(kBlock) void vplus_realloc(int n_cubes, int32_t* __
   }
   ne = block_sum(ne, part);  // an integer < 2^53: exact
   if (threadIdx.x == 0) {
-    out[3 + 3 * (int64_t)nb + blockIdx.x] = ne;
-    if (blockIdx.x == 0) {
+    out[3 + 3 * (int64_t)nb + b] = ne;
+    if (b == 0) {
       out[0] = tot[0];
       out[1] = sqrt(tot[1]);  // vflowplus.py:233
       flag_nonfinite(status, out[0], tot[1]);
       // this iteration's events: offsets are rewritten only by vplus_offsets
       advance_counter(counter, out[0], out[1], (double)offsets[n_cubes]);
+      return true;
     }
   }
+  return false;
+}
+
+__global__ __launch_bounds__(kBlock) void vplus_realloc(int n_cubes, int32_t* __restrict__ n_ev,
+                                                        const int32_t* __restrict__ offsets,
+                                                        const double* __restrict__ ress2, int adaptive,
+                                                        int min_neval, double init_calls,
+                                                        double* __restrict__ out, CounterK counter,
+                                                        uint32_t* status) {
+  realloc_body(blockIdx.x, n_cubes, n_ev, offsets, ress2, adaptive, min_neval, init_calls, out, counter,
+               status);
 }
 
 // Part 3: offsets of the next iteration, the exclusive prefix sum of n_ev
 // (vflowplus.py:67), out[2] = events of the next iteration, ress and ress2
 // zeroed. Workgroups beyond the cubes' (with train): the histogram rows of one
 // dimension each and the grid refinement (vflow.py:349-362).
+__device__ inline void offsets_body(int b, int n_cubes, const int32_t* __restrict__ n_ev,
+                                    int32_t* __restrict__ off, double* __restrict__ ress,
+                                    double* __restrict__ ress2, double* __restrict__ out) {
+  const int nb = (n_cubes + kBlock - 1) / kBlock;
+  __shared__ double part[kWaves];
+  const double* bt = out + 3 + 3 * (int64_t)nb;
+  // events of the workgroups before this one (integers: order-free, exact)
+  double before = 0.0;
+  for (int k = threadIdx.x; k < b; k += kBlock) before += bt[k];
+  before = block_sum(before, part);
+  const int c = b * kBlock + threadIdx.x;
+  const int64_t v = c < n_cubes ? n_ev[c] : 0;
+  int64_t total = 0;
+  const int64_t ex = block_exclusive_scan(v, &total) + (int64_t)before;
+  if (c < n_cubes) {
+    off[c] = (int32_t)ex;
+    ress[c] = 0.0;
+    ress2[c] = 0.0;
+  }
+  if (b == nb - 1 && threadIdx.x == 0) {
+    off[n_cubes] = (int32_t)((int64_t)before + total);
+    out[2] = before + (double)total;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void vplus_offsets(int n_cubes, const int32_t* __restrict__ n_ev,
                                                         int32_t* __restrict__ off, double* __restrict__ ress,
                                                         double* __restrict__ ress2, double* __restrict__ out,
@@ -290,25 +337,83 @@ __global__ __launch_bounds__(kBlock) void vplus_offsets(int n_cubes, const int32
     hist_refine_block(z, P, stride, nrows, (int)blockIdx.x - nb, grid, hist_out);
     return;
   }
-  __shared__ double part[kWaves];
-  const double* bt = out + 3 + 3 * (int64_t)nb;
-  // events of the workgroups before this one (integers: order-free, exact)
-  double before = 0.0;
-  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kBlock) before += bt[b];
-  before = block_sum(before, part);
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  const int64_t v = c < n_cubes ? n_ev[c] : 0;
-  int64_t total = 0;
-  const int64_t ex = block_exclusive_scan(v, &total) + (int64_t)before;
-  if (c < n_cubes) {
-    off[c] = (int32_t)ex;
-    ress[c] = 0.0;
-    ress2[c] = 0.0;
+  offsets_body(blockIdx.x, n_cubes, n_ev, off, ress, ress2, out);
+}
+
+// ---------------------------------------------------------------------------
+// batched VegasFlowPlus iteration end (mv_batch_vplus_finalize): the three
+// kernels above for many items, one workgroup per {item, chunk of 256 cubes}
+// job, and in the third launch one more per {item, dimension} job for the
+// histogram columns and the refinement.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void batch_vplus_stats(const mv_batch_vplus_item* __restrict__ items,
+                                                            const int32_t* __restrict__ chunks, double beta,
+                                                            const int32_t* __restrict__ n_ev,
+                                                            const double* __restrict__ ress,
+                                                            double* __restrict__ ress2,
+                                                            double* __restrict__ outs) {
+  const int item = chunks[2 * blockIdx.x], b = chunks[2 * blockIdx.x + 1];
+  const mv_batch_vplus_item& it = items[item];
+  cube_stats_body(b, it.n_cubes, n_ev + it.cube_offset, ress + it.cube_offset, ress2 + it.cube_offset,
+                  it.adaptive, beta, outs + it.out_offset, nullptr);
+}
+
+__global__ __launch_bounds__(kBlock) void batch_vplus_realloc(const mv_batch_vplus_item* __restrict__ items,
+                                                              const int32_t* __restrict__ chunks,
+                                                              int32_t* __restrict__ n_ev,
+                                                              const int32_t* __restrict__ offsets,
+                                                              const double* __restrict__ ress2,
+                                                              double* __restrict__ outs,
+                                                              int32_t* __restrict__ draws,
+                                                              double* __restrict__ stats) {
+  const int item = chunks[2 * blockIdx.x], b = chunks[2 * blockIdx.x + 1];
+  const mv_batch_vplus_item& it = items[item];
+  double* out = outs + it.out_offset;
+  if (realloc_body(b, it.n_cubes, n_ev + it.cube_offset, offsets + it.off_offset, ress2 + it.cube_offset,
+                   it.adaptive, it.min_neval, it.init_calls, out, CounterK{}, nullptr)) {
+    // the item's running weighted average (monte_carlo.py:713-732) and its
+    // iteration counter
+    const double res = out[0], sig = out[1];
+    double* st = stats + 4 * (int64_t)item;
+    const double wgt = 1.0 / (sig * sig);  // monte_carlo.py:718
+    st[0] = res;
+    st[1] = sig;
+    st[2] += wgt;        // monte_carlo.py:720
+    st[3] += res * wgt;  // monte_carlo.py:719
+    draws[item] = draws[item] + 1;
   }
-  if ((int)blockIdx.x == nb - 1 && threadIdx.x == 0) {
-    off[n_cubes] = (int32_t)((int64_t)before + total);
-    out[2] = before + (double)total;
+}
+
+__global__ __launch_bounds__(kBlock) void batch_vplus_offsets(const mv_batch_vplus_item* __restrict__ items,
+                                                              const int32_t* __restrict__ chunks, int n_chunks,
+                                                              const int32_t* __restrict__ dims,
+                                                              const int32_t* __restrict__ n_ev,
+                                                              int32_t* __restrict__ offsets,
+                                                              double* __restrict__ ress,
+                                                              double* __restrict__ ress2,
+                                                              double* __restrict__ outs,
+                                                              double* __restrict__ grids,
+                                                              double* __restrict__ rows) {
+  if ((int)blockIdx.x < n_chunks) {
+    const int item = chunks[2 * blockIdx.x], b = chunks[2 * blockIdx.x + 1];
+    const mv_batch_vplus_item& it = items[item];
+    offsets_body(b, it.n_cubes, n_ev + it.cube_offset, offsets + it.off_offset, ress + it.cube_offset,
+                 ress2 + it.cube_offset, outs + it.out_offset);
+    return;
   }
+  const int job = (int)blockIdx.x - n_chunks;
+  const int item = dims[2 * job], dim = dims[2 * job + 1];
+  const mv_batch_vplus_item& it = items[item];
+  __shared__ double hb[kBins];
+  double* row = rows + it.row_offset + 2 + (int64_t)dim * kBins;
+  const int t = threadIdx.x;
+  if (t < kBins) {
+    hb[t] = row[t];
+    row[t] = 0.0;
+  }
+  __syncthreads();
+  if (it.train && t < 64 && wave_hist_finite(hb))
+    refine_wave(hb, grids + it.grid_offset + (int64_t)dim * (kBins + 1));  // vflowplus.py:239-240
 }
 
 }  // namespace mv
@@ -441,4 +546,28 @@ extern "C" int mv_vplus_finalize(int32_t n_dim, const mv_strata* strata, int32_t
   hipLaunchKernelGGL(vplus_offsets, dim3(nb + (train ? n_dim : 0)), dim3(kBlock), 0, s, n_cubes, n_ev,
                      offsets, ress, ress2, out, z, rows, row_stride, n_rows, grid, hist_out);
   return check_launch("mv_vplus_finalize");
+}
+
+extern "C" int mv_batch_vplus_finalize(const mv_batch_vplus_item* items, const int32_t* chunks, int32_t n_chunks,
+                                       const int32_t* dims, int32_t n_dims, double beta, double* grids,
+                                       int32_t* n_ev, int32_t* offsets, double* ress, double* ress2, double* rows,
+                                       double* outs, int32_t* draws, double* stats, void* stream) {
+  if (n_chunks < 0 || n_dims < 0) return fail(MV_ERR_ARG, "mv_batch_vplus_finalize: negative job count");
+  if (n_chunks == 0 && n_dims == 0) return MV_OK;
+  if (!items || (n_chunks && !chunks) || (n_dims && !dims) || !grids || !n_ev || !offsets || !ress || !ress2 ||
+      !rows || !outs || !draws || !stats)
+    return fail(MV_ERR_ARG, "mv_batch_vplus_finalize: missing buffer");
+  hipStream_t s = as_stream(stream);
+  int rc;
+  if (n_chunks) {
+    hipLaunchKernelGGL(batch_vplus_stats, dim3(n_chunks), dim3(kBlock), 0, s, items, chunks, beta, n_ev, ress,
+                       ress2, outs);
+    if ((rc = check_launch("mv_batch_vplus_finalize"))) return rc;
+    hipLaunchKernelGGL(batch_vplus_realloc, dim3(n_chunks), dim3(kBlock), 0, s, items, chunks, n_ev, offsets,
+                       ress2, outs, draws, stats);
+    if ((rc = check_launch("mv_batch_vplus_finalize"))) return rc;
+  }
+  hipLaunchKernelGGL(batch_vplus_offsets, dim3(n_chunks + n_dims), dim3(kBlock), 0, s, items, chunks, n_chunks,
+                     dims, n_ev, offsets, ress, ress2, outs, grids, rows);
+  return check_launch("mv_batch_vplus_finalize");
 }

@@ -63,10 +63,11 @@ __device__ __forceinline__ void trip_cube_sums(int cube, double v, double v2, do
 }
 
 // One event of cube ``cube`` (nev events): returns tmp = w f / nev and, when
-// training, adds tmp^2 to the histograms wh (the workgroup's pair tables, or
-// the wave's per-dimension histogram). INJ: the draws come from s.inject
-// (event i) instead of Philox.
-template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, typename T, typename T2>
+// training, adds tmp^2 to the histograms wh (the workgroup's pair tables if
+// PAIRS, else the wave's per-dimension histogram). INJ: the draws come from
+// s.inject (event i) instead of Philox.
+template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, bool PAIRS = vplus_pairs<TRAIN, D>(),
+          typename T, typename T2>
 __device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK& s, uint32_t draw, int cube,
                                               int nev, int64_t i, int n_strat, double fstrat, double ystrat,
                                               float rn,
@@ -111,7 +112,7 @@ __device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK&
   // vflowplus.py:77, monte_carlo.py:270-274, vflowplus.py:209
   const double wev = (double)w / (double)nev;
   const double tmp = (wev * wscale) * (double)F.value();
-  if (vplus_pairs<TRAIN, D>())
+  if (TRAIN && PAIRS)
     pair_adds(wh, pk, D, tmp * tmp);
   else if (TRAIN)
     hist_adds(wh, pk, D, tmp * tmp);
@@ -120,7 +121,8 @@ __device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK&
 
 // The wave's run of events first .. end - 1, 64 per trip, into acc, acc2
 // (per-lane sums) and the per-cube sums ress, ress2.
-template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, typename T, typename T2>
+template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, bool PAIRS = vplus_pairs<TRAIN, D>(),
+          typename T, typename T2>
 __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, uint32_t draw,
                                           const StrataK& st, int64_t first, int64_t end, double fstrat,
                                           double ystrat, float rn, const T2* tab, const T* tw,
@@ -136,8 +138,8 @@ __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, 
     const int cube = window_cube(st.off, st.n_ev, st.n_cubes, c0, (int)(live ? i : end - 1), nev);
     double tmp = 0.0, tmp2 = 0.0;
     if (live) {
-      tmp = vplus_event<FAM, TRAIN, MIXED, D, INJ>(F, s, draw, cube, nev, i, st.n_strat, fstrat, ystrat, rn,
-                                                   tab, tw, wscale, wh);
+      tmp = vplus_event<FAM, TRAIN, MIXED, D, INJ, PAIRS>(F, s, draw, cube, nev, i, st.n_strat, fstrat, ystrat,
+                                                          rn, tab, tw, wscale, wh);
       tmp2 = tmp * tmp;
       acc += tmp;
       acc2 += tmp2;

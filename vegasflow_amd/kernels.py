@@ -420,3 +420,19 @@ def batch_pairs_blocks(family, max_dim, precision="mixed"):
 def batch_finalize(items_dev, jobs, n_jobs, grids, rows, draws, stats, history=None, max_iter=0):
     _abi.call("mv_batch_finalize", _ptr(items_dev), _ptr(jobs), int(n_jobs), _ptr(grids), _ptr(rows),
               _ptr(draws), _ptr(stats), _ptr(history), int(max_iter), _stream())
+
+
+def batch_vplus_fused(items_dev, tiles_dev, n_tiles, precision, family, max_dim, draws, grids, n_ev, offsets,
+                      ress, ress2, rows, blocks):
+    """mv_batch_vplus_fused; every argument is already on the device."""
+    _abi.call("mv_batch_vplus_fused", _ptr(items_dev), _ptr(tiles_dev), int(n_tiles), precision_code(precision),
+              int(family), int(max_dim), _ptr(draws), _ptr(grids), _ptr(n_ev), _ptr(offsets), _ptr(ress),
+              _ptr(ress2), _ptr(rows), int(blocks), _stream())
+
+
+def batch_vplus_finalize(items_dev, chunks, n_chunks, dims, n_dims, beta, grids, n_ev, offsets, ress, ress2,
+                         rows, outs, draws, stats):
+    """mv_batch_vplus_finalize (three launches); device arguments."""
+    _abi.call("mv_batch_vplus_finalize", _ptr(items_dev), _ptr(chunks), int(n_chunks), _ptr(dims), int(n_dims),
+              float(beta), _ptr(grids), _ptr(n_ev), _ptr(offsets), _ptr(ress), _ptr(ress2), _ptr(rows),
+              _ptr(outs), _ptr(draws), _ptr(stats), _stream())

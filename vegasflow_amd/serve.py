@@ -17,9 +17,16 @@ finite (its integrand gave NaN or Inf) is retired at once. Each ends with
 ``status`` "cancelled", "timeout" or "nonfinite" and its last result;
 converged requests end "converged", those at their iteration cap "max_iter".
 
+Two algorithms share the slots: VegasFlow requests (``algorithm="vegas"``)
+run through ``mv_batch_fused`` / ``mv_batch_fused_pairs`` and
+``mv_batch_finalize``; VegasFlowPlus requests (``algorithm="vegas+"``, the
+stratified sampling with adaptive event allocation of vflowplus.py) through
+``mv_batch_vplus_fused`` and ``mv_batch_vplus_finalize``, with their
+hypercube allocations and per-cube sums resident in the slot.
+
 A request's draws depend only on its seed, its iteration and its event
 index, so its result is the same alone and in the mix up to summation order
-(``run_alone`` reproduces it with VegasFlow).
+(``run_alone`` reproduces it with VegasFlow or VegasFlowPlus).
 """
 import collections
 import ctypes
@@ -30,14 +37,21 @@ import numpy as np
 import torch
 
 from . import _abi, kernels
-from .configflow import BINS_MAX, DTYPE, default_device, precision_of
+from .configflow import BETA, BINS_MAX, DTYPE, MAX_NEVAL_HCUBE, default_device, precision_of
 from .integrands import BuiltinIntegrand
 
 MAX_DIM = _abi.FUSED_MAX_DIM
 GRID_STRIDE = MAX_DIM * (BINS_MAX + 1)
 ROW_STRIDE = 2 + MAX_DIM * BINS_MAX
 ITEM_BYTES = ctypes.sizeof(_abi.BatchItem)
+VP_ITEM_BYTES = ctypes.sizeof(_abi.BatchVplusItem)
 TILE_BYTES = ctypes.sizeof(_abi.BatchTile)
+# VEGAS+ slot arrays: the stratification never makes more than
+# MAX_NEVAL_HCUBE hypercubes (vflowplus.py:115-123)
+MAX_CUBES = MAX_NEVAL_HCUBE
+CUBE_CHUNK = 256  # cubes per workgroup of the VEGAS+ iteration end
+OUT_LEN = 3 + 4 * ((MAX_CUBES + CUBE_CHUNK - 1) // CUBE_CHUNK)  # mv_vplus_out_len(MAX_CUBES)
+ALGORITHMS = ("vegas", "vegas+")
 
 
 @dataclasses.dataclass
@@ -51,6 +65,8 @@ class Request:
     arrival: float = 0.0
     rid: int = -1
     timeout: float = None  # seconds after arrival (None: no limit)
+    algorithm: str = "vegas"  # "vegas" (VegasFlow) or "vegas+" (VegasFlowPlus)
+    adaptive: bool = True  # VegasFlowPlus: reallocate events across hypercubes
     # filled by the server
     admitted: float = None
     done: float = None
@@ -66,8 +82,11 @@ class Request:
 
 
 def slot_bytes():
-    """Device bytes one slot holds (item, grid, row, counters)."""
-    return ITEM_BYTES + 8 * (GRID_STRIDE + ROW_STRIDE) + 4 + 8 * 4
+    """Device bytes one slot holds: both item kinds, grid, row, counters and
+    the VEGAS+ arrays (events and offsets per hypercube, per-cube sums, the
+    iteration end's results and scratch), so that either algorithm fits."""
+    vplus = 4 * MAX_CUBES + 4 * (MAX_CUBES + 1) + 16 * MAX_CUBES + 8 * OUT_LEN
+    return ITEM_BYTES + VP_ITEM_BYTES + 8 * (GRID_STRIDE + ROW_STRIDE) + 4 + 8 * 4 + vplus
 
 
 class Server:
@@ -98,6 +117,15 @@ class Server:
         self.rows = torch.zeros(S * ROW_STRIDE, dtype=torch.float64, device=self.device)
         self.draws = torch.zeros(S, dtype=torch.int32, device=self.device)
         self.stats = torch.zeros(4 * S, dtype=torch.float64, device=self.device)
+        # VegasFlowPlus slots: item, events and offsets per hypercube, per-cube
+        # sums, iteration-end results (slot s owns entries s * MAX_CUBES ..)
+        self.vp_items_host = (_abi.BatchVplusItem * S)()
+        self.vp_items = torch.zeros(S * VP_ITEM_BYTES, dtype=torch.uint8, device=self.device)
+        self.n_ev = torch.zeros(S * MAX_CUBES, dtype=torch.int32, device=self.device)
+        self.offsets = torch.zeros(S * (MAX_CUBES + 1), dtype=torch.int32, device=self.device)
+        self.ress = torch.zeros(S * MAX_CUBES, dtype=torch.float64, device=self.device)
+        self.ress2 = torch.zeros(S * MAX_CUBES, dtype=torch.float64, device=self.device)
+        self.outs = torch.zeros(S * OUT_LEN, dtype=torch.float64, device=self.device)
         self._linspace = torch.linspace(0, 1, BINS_MAX + 1, dtype=torch.float64, device=self.device)
         self.free = list(range(S - 1, -1, -1))
         self.live = {}  # slot -> Request
@@ -116,10 +144,16 @@ class Server:
             raise ValueError(f"requests need a built-in integrand with n_dim <= {MAX_DIM}")
         if req.n_events < 2 or req.n_events >= 1 << 32:
             raise ValueError("a request needs 2 .. 2^32 - 1 events per iteration")
+        if req.algorithm not in ALGORITHMS:
+            raise ValueError(f"unknown algorithm {req.algorithm!r}; one of {ALGORITHMS}")
+        if req.algorithm == "vegas+" and req.n_events >= 1 << 31:
+            raise ValueError("a VegasFlowPlus request needs fewer than 2^31 events per iteration")
         self.queue.append(req)
 
     def _admit(self, req, now):
         s = self.free.pop()
+        if req.algorithm == "vegas+":
+            self._admit_vplus(req, s)
         it = self.items_host[s]
         it.n_dim = req.n_dim
         it.train = 1
@@ -139,6 +173,42 @@ class Server:
         req.status = "running"
         self.live[s] = req
         self._launch = None
+
+    def _admit_vplus(self, req, s):
+        """The VegasFlowPlus state of slot ``s``: stratification, an equal
+        first allocation (vflowplus.py:105-139) and zeroed per-cube sums."""
+        from .vflowplus import stratification
+
+        adaptive, _, n_strat, n_cubes, min_neval = stratification(req.n_dim, req.n_events, req.adaptive)
+        it = self.vp_items_host[s]
+        it.n_dim = req.n_dim
+        it.train = 1
+        it.n_strat = n_strat
+        it.n_cubes = n_cubes
+        it.adaptive = int(adaptive)
+        it.min_neval = min_neval
+        it.init_calls = float(req.n_events)
+        it.seed = req.seed & 0xFFFFFFFFFFFFFFFF
+        it.grid_offset = s * GRID_STRIDE
+        it.row_offset = s * ROW_STRIDE
+        it.cube_offset = s * MAX_CUBES
+        it.off_offset = s * (MAX_CUBES + 1)
+        it.out_offset = s * OUT_LEN
+        it.fn = req.integrand.to_struct(req.n_dim)
+        raw = torch.frombuffer(bytearray(bytes(it)), dtype=torch.uint8)
+        self.vp_items[s * VP_ITEM_BYTES:(s + 1) * VP_ITEM_BYTES].copy_(raw)
+        c0 = s * MAX_CUBES
+        self.n_ev[c0:c0 + n_cubes].fill_(min_neval)
+        o0 = s * (MAX_CUBES + 1)
+        self.offsets[o0:o0 + n_cubes + 1].copy_(
+            torch.arange(n_cubes + 1, dtype=torch.int32, device=self.device) * min_neval)
+        self.ress[c0:c0 + n_cubes].zero_()
+        self.ress2[c0:c0 + n_cubes].zero_()
+        first = n_cubes * min_neval
+        # tiles cover the most events an iteration can have: after a
+        # reallocation sum(max(min, x_c)) <= n_cubes * min + N / 2
+        req._vp = {"n_cubes": n_cubes, "bound": first + (req.n_events // 2 + 1 if adaptive else 0),
+                   "events": first}
 
     def _retire(self, req, now, status):
         """End ``req`` (running or waiting) with ``status``; a slot it held
@@ -167,10 +237,10 @@ class Server:
         return True
 
     def _tiles_of(self, s, req, ept):
-        key = (s, req.n_events, ept)
+        n = req._vp["bound"] if req.algorithm == "vegas+" else req.n_events
+        key = (s, req.algorithm, n, ept)
         t = self._tile_cache.get(key)
         if t is None:
-            n = req.n_events
             firsts = np.arange(0, n, ept, dtype=np.int64)
             counts = np.minimum(ept, n - firsts).astype(np.int32)
             t = np.zeros(len(firsts), dtype=[("item", "<i4"), ("count", "<i4"), ("first", "<i8")])
@@ -186,10 +256,17 @@ class Server:
         list for the current live set (reused by every tick until a request
         is admitted or retired)."""
         by_family = collections.defaultdict(list)
-        jobs = []
+        vp_by_family = collections.defaultdict(list)
+        jobs, vp_chunks, vp_dims = [], [], []
         slots = sorted(self.live)
         for s in slots:
             req = self.live[s]
+            if req.algorithm == "vegas+":
+                vp_by_family[req.integrand.family].append((s, req))
+                nb = (req._vp["n_cubes"] + CUBE_CHUNK - 1) // CUBE_CHUNK
+                vp_chunks.extend((s, b) for b in range(nb))
+                vp_dims.extend((s, j) for j in range(req.n_dim))
+                continue
             by_family[req.integrand.family].append((s, req))
             jobs.append((s, -1))
             jobs.extend((s, j) for j in range(req.n_dim))
@@ -197,7 +274,8 @@ class Server:
         # events, smaller (powers of two) when the live requests are too few to
         # give every workgroup two tiles, so that a light load still spreads
         # over the GPU
-        total = sum(self.live[s].n_events for s in slots)
+        total = sum(self.live[s].n_events for s in slots if self.live[s].algorithm == "vegas")
+        vp_total = sum(self.live[s]._vp["events"] for s in slots if self.live[s].algorithm == "vegas+")
         if len(self._tile_cache) > 4 * self.capacity:
             self._tile_cache.clear()
         fams = []
@@ -213,9 +291,23 @@ class Server:
             tiles = np.concatenate([self._tiles_of(s, r, ept) for s, r in members])
             tiles_dev = torch.from_numpy(tiles.view(np.uint8)).to(self.device)
             fams.append((fam, tiles_dev, len(tiles), max_dim, sum(r.n_events for _, r in members), pb))
-        jobs_dev = torch.tensor(np.array(jobs, dtype=np.int32).reshape(-1), device=self.device)
+        vp_fams = []
+        for fam, members in sorted(vp_by_family.items()):
+            max_dim = max(r.n_dim for _, r in members)
+            ept = self.events_per_tile
+            while ept > 1024 and vp_total < 2 * self.blocks * ept:
+                ept //= 2
+            tiles = np.concatenate([self._tiles_of(s, r, ept) for s, r in members])
+            tiles_dev = torch.from_numpy(tiles.view(np.uint8)).to(self.device)
+            vp_fams.append((fam, tiles_dev, len(tiles), max_dim))
+
+        def dev_pairs(pairs):
+            return torch.tensor(np.array(pairs, dtype=np.int32).reshape(-1), device=self.device)
+
+        jobs_dev = dev_pairs(jobs) if jobs else None
+        vp = (vp_fams, dev_pairs(vp_chunks), len(vp_chunks), dev_pairs(vp_dims), len(vp_dims)) if vp_chunks else None
         slots_dev = torch.tensor(slots, dtype=torch.int64, device=self.device)
-        self._launch = (fams, jobs_dev, len(jobs), slots, slots_dev)
+        self._launch = (fams, jobs_dev, len(jobs), vp, slots, slots_dev)
 
     def tick(self):
         """One iteration for every live slot; returns the requests retired."""
@@ -224,7 +316,7 @@ class Server:
         t_start = time.perf_counter()
         if self._launch is None:
             self._build_launch()
-        fams, jobs_dev, n_jobs, slots, slots_dev = self._launch
+        fams, jobs_dev, n_jobs, vp, slots, slots_dev = self._launch
         for fam, tiles_dev, n_tiles, max_dim, events, pb in fams:
             if pb > 0:
                 kernels.batch_fused_pairs(self.items, self.capacity, tiles_dev, n_tiles, fam, max_dim,
@@ -233,10 +325,22 @@ class Server:
                 kernels.batch_fused(self.items, self.capacity, tiles_dev, n_tiles, self.precision, fam,
                                     max_dim, self.draws, self.grids, self.rows, self.blocks)
             self.events_done += events
-        kernels.batch_finalize(self.items, jobs_dev, n_jobs, self.grids, self.rows, self.draws, self.stats)
-        # the live slots' counters and running sums only (one small copy; synchronises)
+        if n_jobs:
+            kernels.batch_finalize(self.items, jobs_dev, n_jobs, self.grids, self.rows, self.draws, self.stats)
+        if vp is not None:
+            vp_fams, chunks_dev, n_chunks, dims_dev, n_dims = vp
+            for fam, tiles_dev, n_tiles, max_dim in vp_fams:
+                kernels.batch_vplus_fused(self.vp_items, tiles_dev, n_tiles, self.precision, fam, max_dim,
+                                          self.draws, self.grids, self.n_ev, self.offsets, self.ress, self.ress2,
+                                          self.rows, self.blocks)
+            kernels.batch_vplus_finalize(self.vp_items, chunks_dev, n_chunks, dims_dev, n_dims, BETA, self.grids,
+                                         self.n_ev, self.offsets, self.ress, self.ress2, self.rows, self.outs,
+                                         self.draws, self.stats)
+        # the live slots' counters and running sums only, with the VEGAS+
+        # slots' events of the next iteration (one small copy; synchronises)
         live = torch.cat([self.stats.view(-1, 4).index_select(0, slots_dev),
-                          self.draws.index_select(0, slots_dev).to(torch.float64).unsqueeze(1)], 1)
+                          self.draws.index_select(0, slots_dev).to(torch.float64).unsqueeze(1),
+                          self.outs.view(-1, OUT_LEN)[:, 2].index_select(0, slots_dev).unsqueeze(1)], 1)
         rows = live.cpu().numpy()
         now = time.perf_counter()
         self.ticks += 1
@@ -246,6 +350,9 @@ class Server:
             req = self.live[s]
             st = rows[r_i]
             k = int(st[4])
+            if req.algorithm == "vegas+":
+                self.events_done += req._vp["events"]
+                req._vp["events"] = int(st[5])
             with np.errstate(divide="ignore", invalid="ignore"):
                 est = st[3] / st[2]
                 err = np.sqrt(1.0 / st[2])
@@ -281,9 +388,9 @@ class Server:
         request waits for a first-launch load. Leaves the server empty."""
         seen = {}
         for r in requests:
-            seen.setdefault((r.integrand.family, r.n_dim), r)
-        for (_, d), r in seen.items():
-            self.submit(Request(r.integrand, d, 64, target=1.0, max_iter=1, seed=1))
+            seen.setdefault((r.integrand.family, r.n_dim, r.algorithm), r)
+        for (_, d, alg), r in seen.items():
+            self.submit(Request(r.integrand, d, 64, target=1.0, max_iter=1, seed=1, algorithm=alg))
         self.admit_waiting(float("inf"))
         while self.live:
             self.tick()
@@ -320,7 +427,9 @@ def poisson_requests(rate=200.0, duration=60.0, seed=5, events_gauss=(1_000_000,
     ``duration`` seconds; each request is the two-Gaussian mixture in d = 4
     or the Breit-Wigner x power law in d = 6, with a relative-error target
     drawn from ``targets`` (events per iteration: the first entry of the
-    events pair for the looser target, the second for the tighter one)."""
+    events pair for the looser target, the second for the tighter one). The
+    mixture runs as VegasFlowPlus with adaptive allocation (the stratified
+    configuration), the Breit-Wigner as VegasFlow."""
     from .integrands import BreitWignerPower, GaussianMix
 
     rng = np.random.default_rng(seed)
@@ -332,21 +441,27 @@ def poisson_requests(rate=200.0, duration=60.0, seed=5, events_gauss=(1_000_000,
         target = float(targets[int(rng.integers(len(targets)))])
         tight = target == min(targets)
         if rng.random() < 0.5:
-            fn, d, n = GaussianMix(), 4, events_gauss[1 if tight else 0]
+            fn, d, n, alg = GaussianMix(), 4, events_gauss[1 if tight else 0], "vegas+"
         else:
-            fn, d, n = BreitWignerPower(), 6, events_bw[1 if tight else 0]
-        out.append(Request(fn, d, n, target=target, max_iter=max_iter, seed=10_000 + rid, arrival=t, rid=rid))
+            fn, d, n, alg = BreitWignerPower(), 6, events_bw[1 if tight else 0], "vegas"
+        out.append(Request(fn, d, n, target=target, max_iter=max_iter, seed=10_000 + rid, arrival=t, rid=rid,
+                           algorithm=alg))
         rid += 1
     return out
 
 
 def run_alone(req, dtype=None, n_iter=None):
-    """The request on its own through VegasFlow: the same draws, iterations
-    ``n_iter`` (default: what it ran in the server)."""
+    """The request on its own through VegasFlow (or VegasFlowPlus): the same
+    draws, iterations ``n_iter`` (default: what it ran in the server)."""
     from .vflow import VegasFlow
+    from .vflowplus import VegasFlowPlus
 
-    inst = VegasFlow(req.n_dim, req.n_events, seed=req.seed, verbose=False,
-                     dtype=DTYPE if dtype is None else dtype)
+    dtype = DTYPE if dtype is None else dtype
+    if req.algorithm == "vegas+":
+        inst = VegasFlowPlus(req.n_dim, req.n_events, adaptive=req.adaptive, seed=req.seed, verbose=False,
+                             dtype=dtype)
+    else:
+        inst = VegasFlow(req.n_dim, req.n_events, seed=req.seed, verbose=False, dtype=dtype)
     inst.compile(req.integrand)
     return inst.run_integration(n_iter or req.iterations)
 
