@@ -393,12 +393,15 @@ typedef struct {
  * events, e.g. n_cubes * min_neval + init_calls / 2 + 1 when adaptive):
  * draws in the events' hypercubes, grid map, integrand, per-cube sums into
  * ress / ress2 and the (w f)^2 bin histogram into each item's row (columns
- * 2 ..). Every tiled item has integrand family ``family``; max_dim is the
- * largest n_dim among them; ``blocks`` persistent 256-thread workgroups. */
+ * 2 ..). Every tiled item has integrand family ``family``; min_dim and
+ * max_dim bound their n_dim. When 2 <= min_dim and max_dim <= 14 the
+ * histograms are pair tables (dimensions 2p and 2p + 1 share a 50 x 50 table)
+ * in 1024-thread workgroups, at most ``blocks`` and the number resident at
+ * once; otherwise per-wave histograms in ``blocks`` 256-thread workgroups. */
 int mv_batch_vplus_fused(const mv_batch_vplus_item* items, const mv_batch_tile* tiles, int32_t n_tiles,
-                         int32_t precision, int32_t family, int32_t max_dim, const int32_t* draws,
-                         const double* grids, const int32_t* n_ev, const int32_t* offsets, double* ress,
-                         double* ress2, double* rows, int32_t blocks, void* stream);
+                         int32_t precision, int32_t family, int32_t min_dim, int32_t max_dim,
+                         const int32_t* draws, const double* grids, const int32_t* n_ev, const int32_t* offsets,
+                         double* ress, double* ress2, double* rows, int32_t blocks, void* stream);
 
 /* Iteration end of the items (vflowplus.py:222-242, 153-163 and vflow.py:
  * 349-362 per item, as mv_vplus_finalize): ``chunks`` lists device int32

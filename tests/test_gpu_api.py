@@ -248,8 +248,9 @@ def test_vegasflow_iterations_match_oracle(d):
     engine's grid differs from the oracle's after iteration 1 by the
     rounding of the histogram sums (FP64 atomics add in arrival order);
     rebinning turns that into edge shifts of ~1e-15, which move every
-    later point and weight. The bound is the one-iteration bar times the
-    measured growth factor (below 100 for three iterations)."""
+    later point and weight, and can amplify them where a cut falls in a bin
+    of small weight (the batched configuration measured 1.4e-9 after seven
+    iterations); three iterations are held to 1e-9."""
     n, seed = 50000, 77
     fn = lambda x: ora.gaussian(x, 0.1, 0.5)  # noqa: E731
     inst = VegasFlow(d, n, seed=seed, verbose=False)
@@ -266,9 +267,9 @@ def test_vegasflow_iterations_match_oracle(d):
         np.testing.assert_allclose(grids[i + 1], o_grid, rtol=RTOL, atol=1e-15)
     hist, div = ora.run_vegas(fn, d, n, 3, seed)
     for (r, s, _), (orr, os_) in zip(inst.history, hist):
-        assert abs(r - orr) <= 100 * RTOL * abs(orr)
-        assert_sigma(s, os_, kappa_vegas(orr, os_, n), rtol=100 * RTOL)
-    np.testing.assert_allclose(grids[-1], div, rtol=100 * RTOL, atol=1e-14)
+        assert abs(r - orr) <= 1e-9 * abs(orr)
+        assert_sigma(s, os_, kappa_vegas(orr, os_, n), rtol=1e-9)
+    np.testing.assert_allclose(grids[-1], div, rtol=1e-9, atol=1e-12)
 
 
 def test_fused_and_unfused_paths_agree():

@@ -182,8 +182,8 @@ SIGNATURES = {
     "mv_batch_fused_pairs_f64": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "mv_batch_pairs_blocks_f64": (_i32, [_i32, _i32]),
     "mv_batch_finalize": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
-    "mv_batch_vplus_fused": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp,
-                                            _vp, _i32, _vp]),
+    "mv_batch_vplus_fused": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp,
+                                            _vp, _vp, _i32, _vp]),
     "mv_batch_vplus_finalize": (ctypes.c_int, [_vp, _vp, _i32, _vp, _i32, _f64, _vp, _vp, _vp, _vp, _vp, _vp,
                                                _vp, _vp, _vp, _vp]),
 }

@@ -571,3 +571,25 @@ extern "C" int mv_batch_vplus_finalize(const mv_batch_vplus_item* items, const i
                      dims, n_ev, offsets, ress, ress2, outs, grids, rows);
   return check_launch("mv_batch_vplus_finalize");
 }
+
+extern "C" int mv_batch_vplus_fused(const mv_batch_vplus_item* items, const mv_batch_tile* tiles, int32_t n_tiles,
+                                    int32_t precision, int32_t family, int32_t min_dim, int32_t max_dim,
+                                    const int32_t* draws, const double* grids, const int32_t* n_ev,
+                                    const int32_t* offsets, double* ress, double* ress2, double* rows,
+                                    int32_t blocks, void* stream) {
+  if (n_tiles < 0 || blocks < 1) return fail(MV_ERR_ARG, "mv_batch_vplus_fused: bad n_tiles / blocks");
+  if (n_tiles == 0) return MV_OK;
+  if (!items || !tiles || !draws || !grids || !n_ev || !offsets || !ress || !ress2 || !rows)
+    return fail(MV_ERR_ARG, "mv_batch_vplus_fused: missing buffer");
+  if (precision != MV_F64 && precision != MV_MIXED)
+    return fail(MV_ERR_ARG, "mv_batch_vplus_fused: unknown precision %d", precision);
+  if (min_dim < 1 || max_dim < min_dim || max_dim > MV_FUSED_MAX_DIM)
+    return fail(MV_ERR_UNSUPPORTED, "mv_batch_vplus_fused: n_dim range %d..%d outside 1..%d", min_dim, max_dim,
+                MV_FUSED_MAX_DIM);
+  hipStream_t s = as_stream(stream);
+  if (precision == MV_MIXED)
+    return launch_batch_vplus_mixed(items, tiles, n_tiles, family, min_dim, max_dim, draws, grids, n_ev, offsets,
+                                    ress, ress2, rows, blocks, s);
+  return launch_batch_vplus_f64(items, tiles, n_tiles, family, min_dim, max_dim, draws, grids, n_ev, offsets, ress,
+                                ress2, rows, blocks, s);
+}

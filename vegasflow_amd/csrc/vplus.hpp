@@ -140,4 +140,15 @@ int launch_vplus_fused_f64(int family, bool train, const StepK& k, const StrataK
                            const mv_integrand& fn, const double* grid, double* ress, double* ress2,
                            const RowsK& R, hipStream_t s);
 
+// The batched VEGAS+ producer (batch_vplus.hpp) of one precision: 0 or a
+// negative status.
+int launch_batch_vplus_mixed(const mv_batch_vplus_item* items, const mv_batch_tile* tiles, int n_tiles, int family,
+                             int min_dim, int max_dim, const int32_t* draws, const double* grids,
+                             const int32_t* n_ev, const int32_t* offsets, double* ress, double* ress2, double* rows,
+                             int blocks, hipStream_t stream);
+int launch_batch_vplus_f64(const mv_batch_vplus_item* items, const mv_batch_tile* tiles, int n_tiles, int family,
+                             int min_dim, int max_dim, const int32_t* draws, const double* grids,
+                             const int32_t* n_ev, const int32_t* offsets, double* ress, double* ress2, double* rows,
+                             int blocks, hipStream_t stream);
+
 }  // namespace mv

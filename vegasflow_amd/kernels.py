@@ -29,8 +29,57 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+# A recording (see ``recording``) collects the C calls made while it is
+# active, with every tensor and workspace they point into, so that the calls
+# can be issued again with the same arguments.
+_recording = None
+
+
+class Recording:
+    """C calls of the library made while active: ``calls`` holds (function,
+    arguments, name); ``keep`` the tensors and workspaces whose device memory
+    the arguments address (held so that the addresses stay valid)."""
+
+    def __init__(self):
+        self.calls = []
+        self.keep = []
+
+
+class recording:
+    """Context manager: record the library calls made inside it."""
+
+    def __enter__(self):
+        global _recording
+        if _recording is not None:
+            raise RuntimeError("recordings do not nest")
+        _recording = Recording()
+        return _recording
+
+    def __exit__(self, *exc):
+        global _recording
+        _recording = None
+        return False
+
+
+def replay_calls(calls):
+    """Issue recorded calls again, in order, on the streams they named."""
+    for fn, args, name in calls:
+        _abi.check(fn(*args), name)
+
+
+def _call(name, *args):
+    rc = _abi.call(name, *args)
+    if _recording is not None:
+        _recording.calls.append((getattr(_abi.load(), name), args, name))
+    return rc
+
+
 def _ptr(t):
-    return None if t is None else ctypes.c_void_p(t.data_ptr())
+    if t is None:
+        return None
+    if _recording is not None:
+        _recording.keep.append(t)
+    return ctypes.c_void_p(t.data_ptr())
 
 
 def _need_device(t, name, dtype=None):
@@ -85,6 +134,8 @@ class Workspace:
         self._struct = _abi.Partials(self.rows.data_ptr(), self.stride)
 
     def partials(self):
+        if _recording is not None:
+            _recording.keep.append(self)
         return ctypes.byref(self._struct)
 
 
@@ -205,7 +256,7 @@ class IterationCounter:
 def vegas_fused(step, fn_struct, grid, ws):
     """mv_vegas_fused; returns the number of group rows of ``ws`` touched."""
     _need_device(grid, "grid", torch.float64)
-    return _abi.call("mv_vegas_fused", ctypes.byref(step), ctypes.byref(fn_struct), _ptr(grid),
+    return _call("mv_vegas_fused", ctypes.byref(step), ctypes.byref(fn_struct), _ptr(grid),
                      ws.partials(), _stream())
 
 
@@ -220,13 +271,13 @@ def vegas_iteration_static(step, fn_struct, grid, refine, out, hist_out=None, st
     if hist_out is not None and hist_out.numel() < step.n_dim * BINS:
         raise ValueError("hist_out needs n_dim * 50 entries")
     _need_device(status, "status", torch.int32)
-    _abi.call("mv_vegas_iteration_static", ctypes.byref(step), ctypes.byref(fn_struct), _ptr(grid),
+    _call("mv_vegas_iteration_static", ctypes.byref(step), ctypes.byref(fn_struct), _ptr(grid),
               int(bool(refine)), _ptr(out), _ptr(hist_out), _ptr(status), _stream())
     return out
 
 
 def plain_fused(step, fn_struct, ws):
-    return _abi.call("mv_plain_fused", ctypes.byref(step), ctypes.byref(fn_struct), ws.partials(),
+    return _call("mv_plain_fused", ctypes.byref(step), ctypes.byref(fn_struct), ws.partials(),
                      _stream())
 
 
@@ -239,14 +290,14 @@ def vegas_sample(step, grid, x, w, idx=None):
     n, d = step.n_events, step.n_dim
     if x.numel() < n * d or w.numel() < n or (idx is not None and idx.numel() < n * d):
         raise ValueError("sample buffers are too small")
-    _abi.call("mv_vegas_sample", ctypes.byref(step), _ptr(grid), _ptr(x), _ptr(w), _ptr(idx), _stream())
+    _call("mv_vegas_sample", ctypes.byref(step), _ptr(grid), _ptr(x), _ptr(w), _ptr(idx), _stream())
 
 
 def plain_sample(step, x):
     _need_device(x, "x", float_dtype(step.precision))
     if x.numel() < step.n_events * step.n_dim:
         raise ValueError("sample buffer is too small")
-    _abi.call("mv_plain_sample", ctypes.byref(step), _ptr(x), _stream())
+    _call("mv_plain_sample", ctypes.byref(step), _ptr(x), _stream())
 
 
 def vegas_digest(step, f, w, idx, ws, main_dim=0):
@@ -265,7 +316,7 @@ def vegas_digest(step, f, w, idx, ws, main_dim=0):
     w_stride = 0 if w.numel() == 1 else 1
     if w_stride and w.numel() != n:
         raise ValueError("w must have one entry per event or a single entry")
-    return _abi.call("mv_vegas_digest", ctypes.byref(step), _ptr(f), f_stride, n_out, int(main_dim),
+    return _call("mv_vegas_digest", ctypes.byref(step), _ptr(f), f_stride, n_out, int(main_dim),
                      _ptr(w), w_stride, _ptr(idx), ws.partials(), _stream())
 
 
@@ -292,7 +343,9 @@ def vegas_finalize(n_dim, n_out, train, refine, n_events, ws, n_rows, grid=None,
     if status is not None:
         _need_device(status, "status", torch.int32)
         fz.status = status.data_ptr()
-    _abi.call("mv_vegas_finalize", ctypes.byref(fz), _ptr(ws.rows), ws.stride, int(n_rows),
+    if _recording is not None:  # raw addresses in the struct
+        _recording.keep.extend((counter, status))
+    _call("mv_vegas_finalize", ctypes.byref(fz), _ptr(ws.rows), ws.stride, int(n_rows),
               _ptr(grid), _ptr(out), _ptr(hist_out), _stream())
     return out
 
@@ -306,7 +359,7 @@ def refine_grid(hist, grid):
     _need_device(hist, "hist", torch.float64)
     if grid.dim() != 2 or grid.shape[1] != BINS + 1 or hist.shape != (grid.shape[0], BINS):
         raise ValueError("refine_grid expects hist (n_dim, 50) and grid (n_dim, 51)")
-    _abi.call("mv_refine_grid", grid.shape[0], _ptr(hist), BINS, _ptr(grid), _stream())
+    _call("mv_refine_grid", grid.shape[0], _ptr(hist), BINS, _ptr(grid), _stream())
 
 
 def consume_array_into_indices(input_arr, indices, result_size):
@@ -318,7 +371,7 @@ def consume_array_into_indices(input_arr, indices, result_size):
     if inp.numel() != ind.numel():
         raise ValueError("input_arr and indices must have the same number of entries")
     out = torch.empty(int(result_size), dtype=torch.float64, device=inp.device)
-    _abi.call("mv_consume_array_into_indices", _ptr(inp), _ptr(ind), inp.numel(), int(result_size),
+    _call("mv_consume_array_into_indices", _ptr(inp), _ptr(ind), inp.numel(), int(result_size),
               _ptr(out), _stream())
     return out
 
@@ -342,7 +395,7 @@ def vplus_fused(step, strata, fn_struct, grid, ress, ress2, ws):
     _need_device(grid, "grid", torch.float64)
     _need_device(ress, "ress", torch.float64)
     _need_device(ress2, "ress2", torch.float64)
-    return _abi.call("mv_vplus_fused", ctypes.byref(step), ctypes.byref(strata), ctypes.byref(fn_struct),
+    return _call("mv_vplus_fused", ctypes.byref(step), ctypes.byref(strata), ctypes.byref(fn_struct),
                      _ptr(grid), _ptr(ress), _ptr(ress2), ws.partials(), _stream())
 
 
@@ -353,7 +406,7 @@ def vplus_sample(step, strata, grid, x, w, idx, seg):
     _need_device(w, "w", ft)
     _need_device(idx, "idx", torch.uint8)
     _need_device(seg, "seg", torch.int32)
-    _abi.call("mv_vplus_sample", ctypes.byref(step), ctypes.byref(strata), _ptr(grid), _ptr(x), _ptr(w),
+    _call("mv_vplus_sample", ctypes.byref(step), ctypes.byref(strata), _ptr(grid), _ptr(x), _ptr(w),
               _ptr(idx), _ptr(seg), _stream())
 
 
@@ -365,7 +418,7 @@ def vplus_digest(step, strata, f, w, idx, seg, ress, ress2, ws):
     _need_device(seg, "seg", torch.int32)
     if f.dim() != 1:
         raise NotImplementedError("VegasFlowPlus does not support vectorial integrands")
-    return _abi.call("mv_vplus_digest", ctypes.byref(step), ctypes.byref(strata), _ptr(f), _ptr(w),
+    return _call("mv_vplus_digest", ctypes.byref(step), ctypes.byref(strata), _ptr(f), _ptr(w),
                      _ptr(idx), _ptr(seg), _ptr(ress), _ptr(ress2), ws.partials(), _stream())
 
 
@@ -375,9 +428,11 @@ def vplus_finalize(n_dim, strata, n_ev, offsets, ress, ress2, adaptive, min_neva
     """Returns a (3,) float64 tensor: res, sigma, events of the next iteration.
     ``counter``: an IterationCounter, as in vegas_finalize."""
     _need_device(status, "status", torch.int32)
+    if _recording is not None:  # raw addresses below
+        _recording.keep.append(counter)
     out = torch.empty(int(_abi.load().mv_vplus_out_len(int(strata.n_cubes))), dtype=torch.float64,
                       device=ws.device)
-    _abi.call("mv_vplus_finalize", int(n_dim), ctypes.byref(strata), _ptr(n_ev), _ptr(offsets), _ptr(ress),
+    _call("mv_vplus_finalize", int(n_dim), ctypes.byref(strata), _ptr(n_ev), _ptr(offsets), _ptr(ress),
               _ptr(ress2), int(bool(adaptive)), int(min_neval), float(init_calls), float(beta),
               _ptr(ws.rows), ws.stride, int(n_rows), int(bool(train)), int(bool(refine)),
               _ptr(grid), _ptr(out), _ptr(arr_var_out), _ptr(hist_out),
@@ -394,7 +449,7 @@ def batch_fused(items_dev, n_items, tiles_dev, n_tiles, precision, family, max_d
                 blocks):
     """mv_batch_fused; every argument is already on the device (no
     allocation here, so the call can be captured in a HIP graph)."""
-    _abi.call("mv_batch_fused", _ptr(items_dev), int(n_items), _ptr(tiles_dev), int(n_tiles),
+    _call("mv_batch_fused", _ptr(items_dev), int(n_items), _ptr(tiles_dev), int(n_tiles),
               precision_code(precision), int(family), int(max_dim), _ptr(draws), _ptr(grids), _ptr(rows),
               int(blocks), _stream())
 
@@ -404,7 +459,7 @@ def batch_fused_pairs(items_dev, n_items, tiles_dev, n_tiles, family, max_dim, d
     """mv_batch_fused_pairs (mixed) or mv_batch_fused_pairs_f64: pair
     histograms, 1024-thread workgroups; arguments as batch_fused."""
     name = "mv_batch_fused_pairs" if precision_code(precision) == _abi.MIXED else "mv_batch_fused_pairs_f64"
-    _abi.call(name, _ptr(items_dev), int(n_items), _ptr(tiles_dev), int(n_tiles),
+    _call(name, _ptr(items_dev), int(n_items), _ptr(tiles_dev), int(n_tiles),
               int(family), int(max_dim), _ptr(draws), _ptr(grids), _ptr(rows), int(blocks), _stream())
 
 
@@ -418,21 +473,21 @@ def batch_pairs_blocks(family, max_dim, precision="mixed"):
 
 
 def batch_finalize(items_dev, jobs, n_jobs, grids, rows, draws, stats, history=None, max_iter=0):
-    _abi.call("mv_batch_finalize", _ptr(items_dev), _ptr(jobs), int(n_jobs), _ptr(grids), _ptr(rows),
+    _call("mv_batch_finalize", _ptr(items_dev), _ptr(jobs), int(n_jobs), _ptr(grids), _ptr(rows),
               _ptr(draws), _ptr(stats), _ptr(history), int(max_iter), _stream())
 
 
-def batch_vplus_fused(items_dev, tiles_dev, n_tiles, precision, family, max_dim, draws, grids, n_ev, offsets,
-                      ress, ress2, rows, blocks):
+def batch_vplus_fused(items_dev, tiles_dev, n_tiles, precision, family, min_dim, max_dim, draws, grids, n_ev,
+                      offsets, ress, ress2, rows, blocks):
     """mv_batch_vplus_fused; every argument is already on the device."""
-    _abi.call("mv_batch_vplus_fused", _ptr(items_dev), _ptr(tiles_dev), int(n_tiles), precision_code(precision),
-              int(family), int(max_dim), _ptr(draws), _ptr(grids), _ptr(n_ev), _ptr(offsets), _ptr(ress),
-              _ptr(ress2), _ptr(rows), int(blocks), _stream())
+    _call("mv_batch_vplus_fused", _ptr(items_dev), _ptr(tiles_dev), int(n_tiles), precision_code(precision),
+          int(family), int(min_dim), int(max_dim), _ptr(draws), _ptr(grids), _ptr(n_ev), _ptr(offsets), _ptr(ress),
+          _ptr(ress2), _ptr(rows), int(blocks), _stream())
 
 
 def batch_vplus_finalize(items_dev, chunks, n_chunks, dims, n_dims, beta, grids, n_ev, offsets, ress, ress2,
                          rows, outs, draws, stats):
     """mv_batch_vplus_finalize (three launches); device arguments."""
-    _abi.call("mv_batch_vplus_finalize", _ptr(items_dev), _ptr(chunks), int(n_chunks), _ptr(dims), int(n_dims),
-              float(beta), _ptr(grids), _ptr(n_ev), _ptr(offsets), _ptr(ress), _ptr(ress2), _ptr(rows),
-              _ptr(outs), _ptr(draws), _ptr(stats), _stream())
+    _call("mv_batch_vplus_finalize", _ptr(items_dev), _ptr(chunks), int(n_chunks), _ptr(dims), int(n_dims),
+          float(beta), _ptr(grids), _ptr(n_ev), _ptr(offsets), _ptr(ress), _ptr(ress2), _ptr(rows),
+          _ptr(outs), _ptr(draws), _ptr(stats), _stream())

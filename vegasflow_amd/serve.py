@@ -109,7 +109,8 @@ class Server:
             raise ValueError("memory budget below one slot")
         self.min_iter = int(min_iter)
         self.events_per_tile = int(events_per_tile)
-        self.blocks = int(blocks_per_cu) * torch.cuda.get_device_properties(self.device).multi_processor_count
+        self.cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+        self.blocks = int(blocks_per_cu) * self.cus
         S = self.capacity
         self.items_host = (_abi.BatchItem * S)()
         self.items = torch.zeros(S * ITEM_BYTES, dtype=torch.uint8, device=self.device)
@@ -262,7 +263,8 @@ class Server:
         for s in slots:
             req = self.live[s]
             if req.algorithm == "vegas+":
-                vp_by_family[req.integrand.family].append((s, req))
+                # items whose dimensions pair (2..14) share a pair-table launch
+                vp_by_family[(req.integrand.family, 2 <= req.n_dim <= 14)].append((s, req))
                 nb = (req._vp["n_cubes"] + CUBE_CHUNK - 1) // CUBE_CHUNK
                 vp_chunks.extend((s, b) for b in range(nb))
                 vp_dims.extend((s, j) for j in range(req.n_dim))
@@ -292,14 +294,18 @@ class Server:
             tiles_dev = torch.from_numpy(tiles.view(np.uint8)).to(self.device)
             fams.append((fam, tiles_dev, len(tiles), max_dim, sum(r.n_events for _, r in members), pb))
         vp_fams = []
-        for fam, members in sorted(vp_by_family.items()):
+        for (fam, paired), members in sorted(vp_by_family.items()):
+            min_dim = min(r.n_dim for _, r in members)
             max_dim = max(r.n_dim for _, r in members)
-            ept = self.events_per_tile
-            while ept > 1024 and vp_total < 2 * self.blocks * ept:
+            if paired:  # 1024-thread workgroups, about one per CU
+                ept, blocks, floor = self.pair_events_per_tile, self.cus, 4096
+            else:
+                ept, blocks, floor = self.events_per_tile, self.blocks, 1024
+            while ept > floor and vp_total < 2 * blocks * ept:
                 ept //= 2
             tiles = np.concatenate([self._tiles_of(s, r, ept) for s, r in members])
             tiles_dev = torch.from_numpy(tiles.view(np.uint8)).to(self.device)
-            vp_fams.append((fam, tiles_dev, len(tiles), max_dim))
+            vp_fams.append((fam, tiles_dev, len(tiles), min_dim, max_dim))
 
         def dev_pairs(pairs):
             return torch.tensor(np.array(pairs, dtype=np.int32).reshape(-1), device=self.device)
@@ -329,8 +335,8 @@ class Server:
             kernels.batch_finalize(self.items, jobs_dev, n_jobs, self.grids, self.rows, self.draws, self.stats)
         if vp is not None:
             vp_fams, chunks_dev, n_chunks, dims_dev, n_dims = vp
-            for fam, tiles_dev, n_tiles, max_dim in vp_fams:
-                kernels.batch_vplus_fused(self.vp_items, tiles_dev, n_tiles, self.precision, fam, max_dim,
+            for fam, tiles_dev, n_tiles, min_dim, max_dim in vp_fams:
+                kernels.batch_vplus_fused(self.vp_items, tiles_dev, n_tiles, self.precision, fam, min_dim, max_dim,
                                           self.draws, self.grids, self.n_ev, self.offsets, self.ress, self.ress2,
                                           self.rows, self.blocks)
             kernels.batch_vplus_finalize(self.vp_items, chunks_dev, n_chunks, dims_dev, n_dims, BETA, self.grids,
