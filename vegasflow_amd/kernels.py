@@ -242,8 +242,9 @@ class IterationCounter:
         self.history = torch.zeros(3 * self.capacity, dtype=torch.float64, device=device)
 
     def start(self, draw_id):
-        """Next iteration draws ``draw_id`` and lands in history slot 0."""
-        self.draw.copy_(torch.tensor([draw_id, draw_id], dtype=torch.int32))
+        """Next iteration draws ``draw_id`` and lands in history slot 0 (a
+        fill kernel on the current stream: no host-to-device copy)."""
+        self.draw.fill_(int(draw_id))
 
     def results(self, n):
         """(n, 3) device view of the first n iterations' {res, sigma, events}."""
