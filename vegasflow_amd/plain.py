@@ -6,9 +6,8 @@ from .monte_carlo import MonteCarloFlow, sampler, wrapper
 
 
 class PlainFlow(MonteCarloFlow):
-    """
-    Simple Monte Carlo integrator.
-    """
+    """Uniform sampling of the unit hypercube (or the limits): the integral
+    is the mean of f w, with no grid to train (plain.py:11-43)."""
 
     _CAN_RUN_VECTORIAL = True
 
@@ -75,10 +74,10 @@ class PlainFlow(MonteCarloFlow):
 
 
 def plain_wrapper(*args, **kwargs):
-    """Wrapper around PlainFlow"""
+    """``wrapper`` with PlainFlow: (integral, sigma) of n_iter iterations."""
     return wrapper(PlainFlow, *args, **kwargs)
 
 
 def plain_sampler(*args, **kwargs):
-    """Wrapper sampler around PlainFlow"""
+    """``sampler`` with PlainFlow (uniform points, weight 1/n)."""
     return sampler(PlainFlow, *args, **kwargs)

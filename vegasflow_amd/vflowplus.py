@@ -44,9 +44,10 @@ def stratification(n_dim, n_events, adaptive):
 
 
 class VegasFlowPlus(VegasFlow):
-    """
-    Implementation of the VEGAS+ algorithm
-    """
+    """VEGAS+ (vflowplus.py:83-242): the VEGAS grid plus a stratification
+    of the unit hypercube into n_strat^n_dim cubes, each sampled with its own
+    number of events; with ``adaptive`` those numbers follow each cube's
+    variance after every iteration."""
 
     def __init__(self, n_dim, n_events, train=True, adaptive=False, events_limit=None, **kwargs):
         # vflowplus.py:89-99: an iteration is held in one step
