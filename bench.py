@@ -263,7 +263,7 @@ def bench_serving(seconds, dtype, rate=200.0):
     from vegasflow_amd.serve import Server, latency_report, poisson_requests
 
     reqs = poisson_requests(rate=rate, duration=seconds, seed=5)
-    server = Server(memory_budget=256 << 20, dtype=dtype)
+    server = Server(dtype=dtype)  # slab budget from the free device memory
     t0 = time.perf_counter()
     done = server.run_stream(reqs)
     dt = time.perf_counter() - t0
@@ -273,6 +273,7 @@ def bench_serving(seconds, dtype, rate=200.0):
                 "events_per_s": server.events_done / dt, "ticks": server.ticks,
                 "tick_ms_p50": float(np.percentile(ticks, 50)), "tick_ms_p99": float(np.percentile(ticks, 99)),
                 "tick_ms_max": float(ticks.max()), "slots": server.capacity,
+                "slab_budget_bytes": server.memory_budget,
                 "pair_histograms": server.pairs})
     return rep
 
@@ -382,6 +383,37 @@ def calibrate_copy(torch, nbytes=1 << 30, reps=5):
             "ms_per_copy": ms, "GB_per_s_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
+ISSUE_KINDS = ("v_mad_u64_u32", "v_bitop3_b32", "v_fma_f32", "v_mul_f32", "v_fma_f64", "v_add_f64",
+               "v_cvt_f64_f32 + v_mul_f64 + v_cvt_f32_f64 (mean)", "v_exp_f32 + v_mul_f32 (mean)",
+               "v_lshl_add_u32", "Philox4x32-10 block (cycles per block)")
+
+
+def bench_issue_costs(iters=4096):
+    """Shader cycles per wave64 instruction per SIMD of the VALU kinds the
+    fused kernels are made of (libmvprobe.so: independent chains of one
+    kind, 16 waves per SIMD, HIP-event time x the shader clock of the same
+    run), and of one Philox4x32-10 block as the kernels compute it."""
+    import ctypes
+
+    from vegasflow_amd import _build
+
+    path = _build.probe_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.mvp_issue_cost.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp]
+    lib.mvp_issue_cost.restype = ctypes.c_int
+    out = {}
+    for k in range(lib.mvp_kinds()):
+        ms, ghz, per = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        if lib.mvp_issue_cost(k, iters, ctypes.byref(ms), ctypes.byref(ghz), ctypes.byref(per)) != 0:
+            raise RuntimeError(f"issue-cost probe {ISSUE_KINDS[k]} failed")
+        out[ISSUE_KINDS[k]] = round(ms.value * 1e-3 * ghz.value * 1e9 / per.value, 2)
+    out["clock_GHz_last_probe"] = round(ghz.value, 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -394,7 +426,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the VEGAS+, batched and serving runs")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the Poisson stream (60 s: the serving configuration)")
-    ap.add_argument("--only", choices=["unfused", "plain", "vplus", "batch", "serve"],
+    ap.add_argument("--only", choices=["unfused", "plain", "vplus", "batch", "serve", "issue"],
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -421,6 +453,8 @@ def main():
             out = {p: bench_plain(vf, dt_) for p, dt_ in precisions}
         elif args.only == "batch":
             out = {p: bench_batched(torch, dt_) for p, dt_ in precisions}
+        elif args.only == "issue":
+            out = bench_issue_costs()
         else:
             out = {p: bench_serving(args.serve_seconds, dt_) for p, dt_ in precisions}
         print(json.dumps({"section": args.only, "result": out}))

@@ -130,3 +130,39 @@ def test_vplus_requests_converge_where_the_grid_alone_does_not():
     assert vp.status == "converged" and vp.iterations < vg.iterations
     exact = vf.GaussianMix().exact(4)
     assert abs(vp.result - exact) < 4 * vp.error
+
+
+def test_tick_event_cap_bounds_every_tick():
+    """With max_tick_events, a tick's live requests hold at most that many
+    events per iteration (or one request alone), and admission order leaves
+    each result as it is alone."""
+    cap = 150_000
+    server = Server(memory_budget=32 * slot_bytes(), max_tick_events=cap)
+    reqs = mixed_requests(12, seed=4)
+    for r in reqs:
+        r.arrival = 0.0
+    seen = []
+    orig_tick = server.tick
+
+    def tick():
+        seen.append((sum(r.n_events for r in server.live.values()), len(server.live)))
+        assert server.live_events == seen[-1][0]
+        return orig_tick()
+
+    server.tick = tick
+    done = server.run_stream(reqs, warm_up=False)
+    assert len(done) == 12 and server.live_events == 0
+    assert all(ev <= cap or n == 1 for ev, n in seen)
+    assert max(n for _, n in seen) >= 2  # the cap leaves room for several
+    assert any(r.admitted - r.arrival > 0 for r in done)  # and it held some back
+    for req in done[:4]:
+        res, err = run_alone(req)
+        assert abs(req.result - res) <= 1e-9 * abs(res), req.rid
+
+
+def test_default_budget_follows_free_device_memory():
+    before, _ = torch.cuda.mem_get_info()
+    server = Server()
+    after, _ = torch.cuda.mem_get_info()
+    assert min(after // 16, 1 << 30) <= server.memory_budget <= min(before // 16, 1 << 30)
+    assert server.capacity == server.memory_budget // slot_bytes() >= 1

@@ -2,7 +2,9 @@
 
 Every ``csrc/*.hip`` source is compiled by hipcc for gfx950 only and linked
 into ``vegasflow_amd/lib/libmivegas.so`` next to this file, so the library
-travels with the source tree. Flags:
+travels with the source tree. ``csrc/probe/issue_cost.hip``, the measurement
+of VALU issue costs that bench.py reports, becomes ``lib/libmvprobe.so`` of
+its own (``build_probe``). Flags:
 
 - ``-ffp-contract=off``: no implicit FMA contraction, so the index arithmetic
   (xn = 50 * (1 - u), the bin cast) follows the reference's rounding; FMAs
@@ -22,6 +24,8 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libmivegas.so"
+PROBE_SRC = os.path.join(CSRC, "probe", "issue_cost.hip")
+PROBE_LIB = "libmvprobe.so"
 ARCH = "gfx950"
 
 HIP_FLAGS = [
@@ -85,5 +89,25 @@ def build(verbose=False, jobs=None):
     return out
 
 
+def probe_path():
+    return os.path.join(LIBDIR, PROBE_LIB)
+
+
+def build_probe(verbose=False):
+    """Compile and link the issue-cost probe library; returns its path."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = probe_path()
+    tmp = out + ".tmp"
+    cmd = [hipcc(), *HIP_FLAGS, "-shared", PROBE_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {PROBE_SRC}:\n{r.stderr}")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_probe(verbose=True)

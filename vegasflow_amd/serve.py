@@ -2,7 +2,8 @@
 
 Requests (a built-in integrand, n_dim, events per iteration, relative-error
 target, iteration cap, seed) are admitted into slots of a device slab whose
-size follows from a memory budget. Every tick advances all live slots by one
+size follows from a memory budget (by default from the device memory free
+at start), optionally also within a cap on the events of one tick. Every tick advances all live slots by one
 iteration with the batched kernels (``mv_batch_fused`` per family present,
 then ``mv_batch_finalize``): requests that arrived while others run join at
 the next tick, and their work shares the same launches. After each tick the
@@ -89,13 +90,37 @@ def slot_bytes():
     return ITEM_BYTES + VP_ITEM_BYTES + 8 * (GRID_STRIDE + ROW_STRIDE) + 4 + 8 * 4 + vplus
 
 
-class Server:
-    """Request loop over a slab of ``capacity`` slots (from memory_budget)."""
+def free_memory_budget(device, share=16, cap=1 << 30):
+    """A slab budget from the device memory free now: 1/share of it, at most
+    ``cap`` bytes (1 GiB holds 4199 slots)."""
+    free, _ = torch.cuda.mem_get_info(device)
+    return min(int(free) // int(share), int(cap))
 
-    def __init__(self, memory_budget=64 << 20, dtype=None, min_iter=3, events_per_tile=16384,
-                 blocks_per_cu=4, pair_histograms=None, pair_events_per_tile=65536):
+
+class Server:
+    """Request loop over a slab of ``capacity`` slots.
+
+    Admission takes the oldest waiting request while (1) a slot is free: the
+    slab holds ``memory_budget`` bytes (None: ``free_memory_budget``, from
+    the device memory free when the server starts), ``slot_bytes()`` per
+    request whatever its event count, since the fused kernels keep no
+    per-event memory; and (2) with ``max_tick_events`` set, the events of
+    one tick (one iteration of every live request) stay within it, which
+    bounds the time of a tick and so the latency that each admitted request
+    adds to the others. A request larger than the cap is admitted alone.
+    Admission is first come, first served: a request that does not fit
+    holds back the ones behind it."""
+
+    def __init__(self, memory_budget=None, dtype=None, min_iter=3, events_per_tile=16384,
+                 blocks_per_cu=4, pair_histograms=None, pair_events_per_tile=65536,
+                 max_tick_events=None):
         kernels.require_gpu()
         self.device = default_device()
+        if memory_budget is None:
+            memory_budget = free_memory_budget(self.device)
+        self.memory_budget = int(memory_budget)
+        self.max_tick_events = None if max_tick_events is None else int(max_tick_events)
+        self.live_events = 0  # events of one tick over the live requests
         self.precision = precision_of(DTYPE if dtype is None else dtype)
         # training goes through the pair-histogram batch kernel
         # (mv_batch_fused_pairs / _f64) wherever it takes the family's max
@@ -220,6 +245,7 @@ class Server:
         if req.slot in self.live and self.live[req.slot] is req:
             del self.live[req.slot]
             self.free.append(req.slot)
+            self.live_events -= self._tick_events(req)
             self._launch = None
         self.finished.append(req)
 
@@ -379,12 +405,26 @@ class Server:
             retired.append(req)
         return retired
 
+    @staticmethod
+    def _tick_events(req):
+        """Events that ``req`` adds to a tick: its events per iteration (a
+        VegasFlowPlus request's reallocation keeps its total within a few
+        events per hypercube of that)."""
+        return req.n_events
+
     def admit_waiting(self, now):
         while self.queue and self.free and self.queue[0].arrival <= now:
-            req = self.queue.popleft()
+            req = self.queue[0]
             if req.timeout is not None and now - req.arrival >= req.timeout:
+                self.queue.popleft()
                 self._retire(req, now, "timeout")  # expired while waiting
                 continue
+            cost = self._tick_events(req)
+            if self.max_tick_events is not None and self.live and \
+                    self.live_events + cost > self.max_tick_events:
+                break
+            self.queue.popleft()
+            self.live_events += cost
             self._admit(req, now)
 
     # -------------------------------------------------------------- loops
@@ -490,8 +530,10 @@ def latency_report(finished):
             "mean_iterations": float(np.mean([r.iterations for r in rs])),
             "met_target": float(np.mean([_met(r) for r in rs])),
         }
+    waits = np.array([r.admitted - r.arrival for r in finished if r.admitted is not None])
     return {
         "requests": int(lat.size),
+        "queue_wait_ms_p99": float(np.percentile(waits, 99) * 1e3) if waits.size else None,
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "max_ms": float(lat.max() * 1e3),
