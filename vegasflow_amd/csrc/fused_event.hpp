@@ -89,7 +89,7 @@ __device__ __forceinline__ void hist_adds(double* wh, const uint32_t (&pk)[4], i
 // cells, so an event makes one LDS add per pair instead of one per
 // dimension, and the adds of a wave spread over 2500 cells instead of 50,
 // which removes most same-address conflicts of the LDS atomics. The
-// per-dimension histograms are the row and column sums (pair_marginal).
+// per-dimension histograms are the row and column sums (pair_marginal_half).
 // LDS: ceil(d / 2) * 2500 doubles, so only for d <= kPairMaxDim.
 constexpr int kPairBins = kBins * kBins;
 constexpr int kPairMinDim = 2;
@@ -110,15 +110,17 @@ __device__ __forceinline__ void pair_adds(double* H, const uint32_t (&pk)[4], in
   }
 }
 
-// Bin b of dimension j = m / 50 (b = m % 50) from the pair histograms: row
-// b of pair j / 2 for an even j, column b for an odd j, summed in order.
-__device__ __forceinline__ double pair_marginal(const double* H, int m) {
+// Bin b of dimension j = m / 50 (b = m % 50) from the pair histograms is the
+// sum of row b of pair j / 2 for an even j, of column b for an odd j. This is
+// half h of that sum (h = 0: cells 0..24, h = 1: cells 25..49) in order; the
+// bin is half 0 + half 1.
+__device__ __forceinline__ double pair_marginal_half(const double* H, int m, int h) {
   const int j = m / kBins, b = m - j * kBins;
-  const double* c = H + (j >> 1) * kPairBins + ((j & 1) ? b : b * kBins);
   const int stride = (j & 1) ? kBins : 1;
+  const double* c = H + (j >> 1) * kPairBins + ((j & 1) ? b : b * kBins) + h * (kBins / 2) * stride;
   double a = 0.0;
-#pragma unroll 10
-  for (int k = 0; k < kBins; ++k) a += c[k * stride];
+#pragma unroll 5
+  for (int k = 0; k < kBins / 2; ++k) a += c[k * stride];
   return a;
 }
 

@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kPairBlock) void vegas_fused_pairs(StepK s, mv_inte
   const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kPairBlock;
-  const int64_t i0 = (int64_t)blockIdx.x * kPairBlock + threadIdx.x;
+  const int64_t i0 = spread_first_event();
   if (inj) {
     for (int64_t i = i0; i < s.n_events; i += step) {
       const double tmp = event_mixed_pairs<FAM, true>(F, 0u, 0u, 0u, s.key, D, tab, cs, H,

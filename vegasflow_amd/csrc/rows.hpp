@@ -140,6 +140,17 @@ inline int make_rows(const mv_partials* p, int64_t need_stride, const char* what
   return MV_OK;
 }
 
+// First event of this thread in a grid-stride loop whose stride is all the
+// launch's threads (blockDim.x a multiple of 64): the lanes of a wave take 64
+// consecutive events, and wave w of workgroup b takes wave slot
+// w * gridDim.x + b, so consecutive slots lie in different workgroups. The
+// last, partial round of the loop then falls on a few waves of every
+// workgroup (at most one more round per SIMD) instead of on every wave of the
+// first workgroups, whose CUs would finish one round after all the others.
+__device__ __forceinline__ int64_t spread_first_event() {
+  return ((int64_t)(threadIdx.x >> 6) * gridDim.x + blockIdx.x) * 64 + (threadIdx.x & 63);
+}
+
 // this workgroup's group row
 __device__ __forceinline__ double* group_row(const RowsK& R) {
   return R.rows + (int64_t)(blockIdx.x % kRows) * R.stride;
@@ -181,7 +192,7 @@ __device__ __forceinline__ void finish_row(double acc, double acc2, const double
 }
 
 // Workgroup end of the pair-histogram producers (BS threads): sums as in
-// finish_row, histogram bins from the pair tables (pair_marginal). SUMS =
+// finish_row, histogram bins from the pair tables (pair_marginal_half). SUMS =
 // false leaves columns 0 and 1 alone (VEGAS+ sums per hypercube).
 template <int BS, bool SUMS = true>
 __device__ __forceinline__ void finish_pairs(double acc, double acc2, const double* H, int d,
@@ -204,9 +215,12 @@ __device__ __forceinline__ void finish_pairs(double acc, double acc2, const doub
     for (int k = 1; k < W; ++k) a += r[k];
     row_add(&row[threadIdx.x], a);
   }
-  for (int m = threadIdx.x; m < d * kBins; m += BS) {
-    const double h = pair_marginal(H, m);
-    if (h != 0.0) row_add(&row[2 + m], h);
+  // two threads per bin (adjacent lanes), each summing half of its cells
+  for (int t = threadIdx.x; t < 2 * d * kBins; t += BS) {
+    const int m = t >> 1, half = t & 1;
+    const double part = pair_marginal_half(H, m, half);
+    const double h = part + __shfl_xor(part, 1, 64);  // half 0 + half 1 in both lanes
+    if (half == 0 && h != 0.0) row_add(&row[2 + m], h);
   }
 }
 
