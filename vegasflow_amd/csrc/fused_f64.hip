@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kBlock) void vegas_fused_f64(StepK s, mv_integrand 
   const double wscale = s.xmin ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < s.n_events; i += step) {
+  for (int64_t i = spread_first_event(); i < s.n_events; i += step) {
     const double tmp = event_f64<FAM, TRAIN, false, INJ ? 1 : 0>(
         F, s.ev_lo + (uint32_t)i, s.ev_hi, draw, s.key, (D + 3) / 4, D, tab, tw, wh,
         INJ ? inj + i * D : nullptr, wscale);

@@ -92,7 +92,7 @@ def slot_bytes():
 
 def free_memory_budget(device, share=16, cap=1 << 30):
     """A slab budget from the device memory free now: 1/share of it, at most
-    ``cap`` bytes (1 GiB holds 4199 slots)."""
+    ``cap`` bytes (1 GiB holds 4198 slots)."""
     free, _ = torch.cuda.mem_get_info(device)
     return min(int(free) // int(share), int(cap))
 
