@@ -37,9 +37,14 @@ PEAK = {"mixed": 157.3, "f64": 78.6}  # TFLOP/s, MI355X datasheet (FP32 / FP64 v
 TRAFFIC = {"mixed": 0.9748e6, "f64": 0.9830e6}
 TRAFFIC_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_pmc.md"
 # VALU wave instructions per launch (SQ_INSTS_VALU, same profile) and the
-# issue peak they are priced against: one wave64 VALU instruction per SIMD
-# every 4 cycles, 4 SIMDs per CU, 2.4 GHz engine clock (MI355X datasheet)
+# VALU-busy cycles per 64 events: the instruction classes of the counters
+# (profiles/vegas_fused_gauss_d8_1e7_valu_mix.md) priced at the issue costs
+# `bench.py --only issue` measures on gfx950 (2 cycles per wave64 FP32 add,
+# multiply or FMA, 8 per v_exp_f32, 4 for every other kind), set against
+# 4 SIMDs per CU at the 2.4 GHz engine clock (MI355X datasheet)
 VALU_INSTS = {"mixed": 2.535e7, "f64": 5.093e7}
+VALU_BUSY_CYCLES = {"mixed": 601.2, "f64": 1303.8}
+VALU_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_valu_mix.md"
 CLOCK_HZ = 2.4e9
 # host cores one GPU's share of a GPU box holds (worker pools are sized to it)
 CPU_SHARE = 16
@@ -490,12 +495,12 @@ def main():
     flop = flops_per_event(D, True) * N
     achieved = flop / (head["fused_kernel_ms"] * 1e-3) / 1e12
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    valu_peak = cus * 4 * CLOCK_HZ / 4  # wave instructions per second
 
     def valu_issue(prec):
-        rate = VALU_INSTS[prec] / (results[prec]["fused_kernel_ms"] * 1e-3)
-        return {"insts_per_launch": VALU_INSTS[prec], "achieved_G_per_s": rate / 1e9,
-                "peak_G_per_s": valu_peak / 1e9, "frac": rate / valu_peak}
+        simd_cycles = cus * 4 * CLOCK_HZ * results[prec]["fused_kernel_ms"] * 1e-3
+        busy = VALU_BUSY_CYCLES[prec] * N / 64
+        return {"insts_per_launch": VALU_INSTS[prec], "busy_cycles_per_64_events": VALU_BUSY_CYCLES[prec],
+                "busy_frac": busy / simd_cycles, "source": VALU_SOURCE}
 
     line = {
         "metric": "VEGAS events/s, d=8 Gaussian, mixed vs FP64 at equal error",
@@ -539,6 +544,10 @@ def main():
         },
     }
     line["roofline"]["f64_kernel"]["frac"] = line["roofline"]["f64_kernel"]["achieved"] / PEAK["f64"]
+    try:  # the issue costs the busy cycles are priced at, measured on this GPU now
+        line["roofline"]["valu_issue_cycles"] = bench_issue_costs()
+    except (OSError, RuntimeError) as e:
+        line["roofline"]["valu_issue_cycles"] = {"error": str(e)}
     if not args.no_extra:
         line["unfused_torch_integrand"] = {p: bench_unfused(vf, dt_) for p, dt_ in
                                            (("mixed", torch.float32), ("f64", torch.float64))}
