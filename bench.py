@@ -208,8 +208,11 @@ def bench_vplus(vf, dtype, n_iter=10):
            "chi2_dof": float(inst.chi2_dof), "events_per_s": events / dt, "ms_per_iteration": 1e3 * dt / n_iter,
            "events": events, "iterations": n_iter, "events_first_iteration": inst.events_history[0],
            "events_last_iteration": inst.events_history[-1]}
-    # steady state: 50 further iterations of the same instance (recorded
-    # calls issued again; the fixed cost of a run spread over more of them)
+    # steady state: 50 further iterations of the same instance, replayed from
+    # the HIP graph of one iteration (runs of 16 or more iterations capture
+    # one); an untimed run of 50 first takes the capture, and the process's
+    # first graph instantiation (about 4 ms), out of the timed run
+    inst.run_integration(50)
     first = len(inst.events_history)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
