@@ -339,3 +339,20 @@ def test_chi2_per_dof_closed_form():
     vec = [(np.array([1.0, 2.0]), np.array([0.1, 0.1])), (np.array([1.1, 2.0]), np.array([0.1, 0.2]))]
     out = chi2_per_dof(vec, np.array([1.05, 2.0]))
     np.testing.assert_allclose(out, [0.5, 0.0])
+
+
+def test_issue_cost_probe_library_loads():
+    """libmvprobe.so (the issue-cost measurement bench.py reports) is built
+    beside the main library and names one result per bench label; argument
+    errors return before any device call."""
+    import bench
+    from vegasflow_amd import _build
+
+    path = _build.probe_path()
+    if not os.path.exists(path):
+        pytest.skip("probe library not built (run __graft_entry__.build())")
+    probe = ctypes.CDLL(path)
+    assert probe.mvp_kinds() == len(bench.ISSUE_KINDS)
+    d = ctypes.c_double()
+    assert probe.mvp_issue_cost(-1, 4096, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == -1
+    assert probe.mvp_issue_cost(probe.mvp_kinds(), 4096, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == -1
