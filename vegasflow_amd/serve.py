@@ -3,10 +3,11 @@
 Requests (a built-in integrand, n_dim, events per iteration, relative-error
 target, iteration cap, seed) are admitted into slots of a device slab whose
 size follows from a memory budget (by default from the device memory free
-at start), optionally also within a cap on the events of one tick. Every tick advances all live slots by one
-iteration with the batched kernels (``mv_batch_fused`` per family present,
-then ``mv_batch_finalize``): requests that arrived while others run join at
-the next tick, and their work shares the same launches. After each tick the
+at start), optionally also within a cap on the events of one tick. Every
+tick advances all live slots by one iteration with the batched kernels
+(``mv_batch_fused`` per family present, then ``mv_batch_finalize``):
+requests that arrived while others run join at the next tick, and their
+work shares the same launches. After each tick the
 host reads the per-slot iteration counters and running weighted averages,
 retires the requests that reached their target (after ``min_iter``
 iterations) or their cap, and admits waiting requests into the freed slots.
@@ -424,8 +425,8 @@ class Server:
                     self.live_events + cost > self.max_tick_events:
                 break
             self.queue.popleft()
-            self.live_events += cost
             self._admit(req, now)
+            self.live_events += cost
 
     # -------------------------------------------------------------- loops
     def warm_up(self, requests):
