@@ -40,3 +40,26 @@ def assert_sigma(got, want, kappa, rtol=RTOL):
     """|got - want| <= rtol * kappa * |want|."""
     got, want = float(got), float(want)
     assert abs(got - want) <= rtol * kappa * abs(want), (got, want, kappa, abs(got - want) / abs(want))
+
+
+def sigma_noise_floor(res, n_events):
+    """The sigma that rounding alone leaves when every tmp is equal (true
+    sigma 0, e.g. a constant integrand): N S2 + S1^2 = 2 S1^2 rounded at
+    2^-52 twice gives sigma^2 <= 4 eps S1^2 / (N - 1); twice that bound."""
+    return 2.0 * math.sqrt(4.0 * 2.0 ** -52 / (float(n_events) - 1.0)) * abs(float(res))
+
+
+def assert_history(got, want, events, rtol):
+    """Per-iteration (integral, sigma) histories of integrals with the given
+    event counts: integrals to rtol, sigma to rtol * kappa (or, where one of
+    them is zero, within the rounding floor of a zero sigma)."""
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.shape == want.shape
+    for item in range(want.shape[0]):
+        for k in range(want.shape[1]):
+            (r, s), (wr, ws) = got[item, k], want[item, k]
+            assert abs(r - wr) <= rtol * abs(wr), (item, k, r, wr)
+            if ws == 0.0 or s == 0.0:
+                assert max(s, ws) <= sigma_noise_floor(wr, events[item]), (item, k, s, ws)
+            else:
+                assert_sigma(s, ws, kappa_vegas(wr, ws, events[item]), rtol)

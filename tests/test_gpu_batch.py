@@ -11,7 +11,7 @@ import torch
 
 import vegasflow_amd as vf
 from oracle import vegas as ora
-from tests.parity import RTOL, assert_sigma, kappa_vegas
+from tests.parity import RTOL, assert_history, assert_sigma, kappa_vegas
 from vegasflow_amd import autotune, kernels
 from vegasflow_amd.batch import BatchVegas, random_batch
 
@@ -41,9 +41,8 @@ def test_batch_items_match_lone_runs(dtype):
         lone = vf.VegasFlow(d, n, seed=seed, verbose=False, dtype=dtype)
         lone.compile(fn)
         lone.run_integration(3)
-        for k, (r, s, _) in enumerate(lone.history):
-            assert abs(hist[i, k, 0] - r) <= 1e-10 * abs(r) + 1e-300, (i, k)
-            assert abs(hist[i, k, 1] - s) <= 1e-7 * abs(s) + 1e-300, (i, k)
+        want = np.array([[(r, s) for r, s, _ in lone.history]])
+        assert_history(hist[i:i + 1], want, [n], rtol=1e-10)
         np.testing.assert_allclose(b.grid(i).cpu().numpy(), lone.divisions.cpu().numpy(),
                                    rtol=1e-10, atol=1e-13)
     np.testing.assert_array_equal(b.iterations(), 3)
@@ -76,9 +75,8 @@ def test_batch_pair_histograms_match_lone_runs(dtype):
         lone = vf.VegasFlow(d, n, seed=seed, verbose=False, dtype=dtype)
         lone.compile(fn)
         lone.run_integration(3)
-        for k, (r, s, _) in enumerate(lone.history):
-            assert abs(hist[i, k, 0] - r) <= 1e-10 * abs(r) + 1e-300, (i, k)
-            assert abs(hist[i, k, 1] - s) <= 1e-7 * abs(s) + 1e-300, (i, k)
+        want = np.array([[(r, s) for r, s, _ in lone.history]])
+        assert_history(hist[i:i + 1], want, [n], rtol=1e-10)
         np.testing.assert_allclose(b.grid(i).cpu().numpy(), lone.divisions.cpu().numpy(),
                                    rtol=1e-10, atol=1e-13)
 
@@ -93,7 +91,7 @@ def test_batch_pair_kernel_frozen_items_match_per_wave_kernel(dtype):
     b = BatchVegas(specs, dtype=dtype, train=False, events_per_tile=65536, pair_histograms=True)
     a.run(2, graph=False)
     b.run(2, graph=False)
-    np.testing.assert_allclose(b.history(), a.history(), rtol=1e-10)
+    assert_history(b.history(), a.history(), [n for _, _, n, _ in specs], rtol=1e-10)
 
 
 def test_batch_graph_replays_match_eager_rounds():

@@ -83,17 +83,42 @@ __device__ __forceinline__ int mixed_position(uint32_t hw, float& one_plus_frac)
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
-
 __device__ __forceinline__ double bcast_lane(double v, int lane) {
   // lane is wave-uniform
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
   return __hiloint2double(hi, lo);
+}
+
+// v of the lane CTRL / ROW_MASK selects through a DPP move (rows not in
+// ROW_MASK read 0), both 32-bit halves
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  int lo, hi;
+  if (ROW_MASK == 0xF) {  // every lane written: no old value
+    lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+  }
+  return __hiloint2double(hi, lo);
+}
+
+// Sum of v over the 64 lanes of a wave, returned in every lane; the whole
+// wave must be active. Data moves by DPP within the VALU (no LDS round trip):
+// lane pairs and quads (quad_perm), the 16 lanes of a row (row_ror 4, 8),
+// then rows 0 + 1 and 2 + 3 (row_bcast 15 into rows 1 and 3) and all four
+// (row_bcast 31 into rows 2 and 3); the total ends in lane 63. A fixed
+// order of additions.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x124, 0xF>(v);  // row_ror 4
+  v += dpp_f64<0x128, 0xF>(v);  // row_ror 8
+  v += dpp_f64<0x142, 0xA>(v);  // row_bcast 15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xC>(v);  // row_bcast 31 into rows 2 and 3
+  return bcast_lane(v, 63);
 }
 
 // LDS float add without return (ds_add_f64 / ds_add_f32 on gfx950).

@@ -11,10 +11,12 @@
 //
 // Every wave owns one contiguous run of the iteration's events, taken 64 at
 // a time. The run's first cube comes from one binary search over the
-// offsets; each later trip starts in the cube where the previous one ended,
-// so a lane finds its cube by a lifting over the next 64 offsets only. Per
-// cube sums: one wave reduction when the trip lies in a single cube, else a
-// segmented scan; one FP64 global atomic per (trip, cube) segment.
+// offsets; each later trip starts in the cube where the previous one ended.
+// A trip inside one cube needs no per-lane lookup, and its sums stay in the
+// lanes until the wave leaves the cube (one wave reduction and one FP64
+// global atomic pair per cube and wave); a trip across cube boundaries finds
+// each lane's cube by a lifting over the next 64 offsets and adds its sums
+// by a segmented scan, one atomic pair per (trip, cube) segment.
 #pragma once
 
 #include <array>
@@ -119,8 +121,26 @@ __device__ __forceinline__ double vplus_event(Integrand<T, FAM>& F, const StepK&
   return tmp;
 }
 
+// Per-cube sums that the lanes of a wave hold for hypercube ``cube`` (-1:
+// none): one wave reduction and one atomic pair, then cleared.
+__device__ __forceinline__ void flush_held_cube(int cube, double& v, double& v2, double* __restrict__ ress,
+                                                double* __restrict__ ress2) {
+  if (cube < 0) return;  // wave-uniform
+  const double a = wave_sum(v), a2 = wave_sum(v2);
+  if ((threadIdx.x & 63) == 0) {
+    unsafeAtomicAdd(&ress[cube], a);
+    unsafeAtomicAdd(&ress2[cube], a2);
+  }
+  v = v2 = 0.0;
+}
+
 // The wave's run of events first .. end - 1, 64 per trip, into acc, acc2
-// (per-lane sums) and the per-cube sums ress, ress2.
+// (per-lane sums) and the per-cube sums ress, ress2 (vflowplus.py:213-214).
+// A trip that lies inside one hypercube (the common case when cubes hold
+// more than 64 events) takes its cube and n_ev from two scalar reads and
+// keeps the cube's sums in the lanes until the wave moves on to another
+// cube; a trip across cube boundaries finds each lane's cube with
+// window_cube and adds its per-cube sums at once (trip_cube_sums).
 template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, bool PAIRS = vplus_pairs<TRAIN, D>(),
           typename T, typename T2>
 __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, uint32_t draw,
@@ -130,12 +150,21 @@ __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, 
                                           double* __restrict__ ress2,
                                           double& acc, double& acc2) {
   const int lane = threadIdx.x & 63;
-  int c0 = first < end ? find_cube(st.off, st.n_cubes, (int)first) : 0;
+  // c0: the cube of the trip's first event (wave-uniform); c0_end: its end
+  int c0 = first < end ? __builtin_amdgcn_readfirstlane(find_cube(st.off, st.n_cubes, (int)first)) : 0;
+  int64_t c0_end = first < end ? (int64_t)st.off[c0 + 1] : 0;
+  int held = -1;  // the cube whose sums the lanes hold in hv, hv2
+  double hv = 0.0, hv2 = 0.0;
   for (int64_t base = first; base < end; base += 64) {
+    while (c0_end <= base) c0_end = (int64_t)st.off[++c0 + 1];  // cubes are never empty
     const int64_t i = base + lane;
     const bool live = i < end;
-    int nev;
-    const int cube = window_cube(st.off, st.n_ev, st.n_cubes, c0, (int)(live ? i : end - 1), nev);
+    const bool inside = min(base + 63, end - 1) < c0_end;  // the whole trip lies in cube c0
+    int cube = c0, nev;
+    if (inside)
+      nev = st.n_ev[c0];
+    else
+      cube = window_cube(st.off, st.n_ev, st.n_cubes, c0, (int)(live ? i : end - 1), nev);
     double tmp = 0.0, tmp2 = 0.0;
     if (live) {
       tmp = vplus_event<FAM, TRAIN, MIXED, D, INJ, PAIRS>(F, s, draw, cube, nev, i, st.n_strat, fstrat, ystrat,
@@ -144,10 +173,21 @@ __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, 
       acc += tmp;
       acc2 += tmp2;
     }
-    // the next trip starts in the cube of this trip's last event
-    c0 = __builtin_amdgcn_readlane(cube, 63);
-    trip_cube_sums(cube, tmp, tmp2, ress, ress2);
+    if (inside) {
+      if (held != c0) {
+        flush_held_cube(held, hv, hv2, ress, ress2);
+        held = c0;
+      }
+      hv += tmp;
+      hv2 += tmp2;
+    } else {
+      trip_cube_sums(cube, tmp, tmp2, ress, ress2);
+      // the next trip starts in the cube of this trip's last event or later
+      c0 = __builtin_amdgcn_readlane(cube, 63);
+      c0_end = (int64_t)st.off[c0 + 1];
+    }
   }
+  flush_held_cube(held, hv, hv2, ress, ress2);
 }
 
 }  // namespace
