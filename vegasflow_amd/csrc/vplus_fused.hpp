@@ -12,11 +12,12 @@
 // Every wave owns one contiguous run of the iteration's events, taken 64 at
 // a time. The run's first cube comes from one binary search over the
 // offsets; each later trip starts in the cube where the previous one ended.
-// A trip inside one cube needs no per-lane lookup, and its sums stay in the
-// lanes until the wave leaves the cube (one wave reduction and one FP64
-// global atomic pair per cube and wave); a trip across cube boundaries finds
-// each lane's cube by a lifting over the next 64 offsets and adds its sums
-// by a segmented scan, one atomic pair per (trip, cube) segment.
+// A trip inside one cube or across one cube boundary needs no per-lane
+// lookup, and its sums stay in the lanes until the wave leaves the cube (one
+// wave reduction and one FP64 global atomic pair per cube and wave); a trip
+// across more cubes finds each lane's cube by a lifting over the next 64
+// offsets and adds its sums by a segmented scan, one atomic pair per (trip,
+// cube) segment.
 #pragma once
 
 #include <array>
@@ -136,10 +137,10 @@ __device__ __forceinline__ void flush_held_cube(int cube, double& v, double& v2,
 
 // The wave's run of events first .. end - 1, 64 per trip, into acc, acc2
 // (per-lane sums) and the per-cube sums ress, ress2 (vflowplus.py:213-214).
-// A trip that lies inside one hypercube (the common case when cubes hold
-// more than 64 events) takes its cube and n_ev from two scalar reads and
-// keeps the cube's sums in the lanes until the wave moves on to another
-// cube; a trip across cube boundaries finds each lane's cube with
+// A trip that lies inside one hypercube, or across the boundary of two,
+// takes cubes and n_ev from scalar reads and keeps the current cube's sums
+// in the lanes until the wave moves on to another cube (one reduction per
+// cube and wave); a trip across more cubes finds each lane's cube with
 // window_cube and adds its per-cube sums at once (trip_cube_sums).
 template <int FAM, bool TRAIN, bool MIXED, int D, bool INJ, bool PAIRS = vplus_pairs<TRAIN, D>(),
           typename T, typename T2>
@@ -159,12 +160,24 @@ __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, 
     while (c0_end <= base) c0_end = (int64_t)st.off[++c0 + 1];  // cubes are never empty
     const int64_t i = base + lane;
     const bool live = i < end;
-    const bool inside = min(base + 63, end - 1) < c0_end;  // the whole trip lies in cube c0
+    // the trip lies in cube c0 (inside), in c0 and c0 + 1 (two: lanes from
+    // event c0_end on are in c0 + 1), or across more cubes
+    const int64_t last = min(base + 63, end - 1);
+    const bool inside = last < c0_end;
+    int64_t c1_end = 0;
+    if (!inside) c1_end = (int64_t)st.off[c0 + 2];  // in bounds: cube c0 + 1 holds event `last`
+    const bool two = !inside && last < c1_end;
+    const bool upper = i >= c0_end;  // lane in c0 + 1 (two)
     int cube = c0, nev;
-    if (inside)
+    if (inside) {
       nev = st.n_ev[c0];
-    else
+    } else if (two) {
+      const int n0 = st.n_ev[c0], n1 = st.n_ev[c0 + 1];
+      cube = upper ? c0 + 1 : c0;
+      nev = upper ? n1 : n0;
+    } else {
       cube = window_cube(st.off, st.n_ev, st.n_cubes, c0, (int)(live ? i : end - 1), nev);
+    }
     double tmp = 0.0, tmp2 = 0.0;
     if (live) {
       tmp = vplus_event<FAM, TRAIN, MIXED, D, INJ, PAIRS>(F, s, draw, cube, nev, i, st.n_strat, fstrat, ystrat,
@@ -173,13 +186,20 @@ __device__ __forceinline__ void vplus_run(Integrand<T, FAM>& F, const StepK& s, 
       acc += tmp;
       acc2 += tmp2;
     }
-    if (inside) {
+    if (inside || two) {
       if (held != c0) {
         flush_held_cube(held, hv, hv2, ress, ress2);
         held = c0;
       }
-      hv += tmp;
-      hv2 += tmp2;
+      hv += upper && two ? 0.0 : tmp;
+      hv2 += upper && two ? 0.0 : tmp2;
+      if (two) {  // c0 ends in this trip: its sums go out, c0 + 1 is held next
+        flush_held_cube(held, hv, hv2, ress, ress2);
+        held = ++c0;
+        c0_end = c1_end;
+        hv = upper ? tmp : 0.0;
+        hv2 = upper ? tmp2 : 0.0;
+      }
     } else {
       trip_cube_sums(cube, tmp, tmp2, ress, ress2);
       // the next trip starts in the cube of this trip's last event or later
