@@ -90,19 +90,27 @@ __device__ __forceinline__ double bcast_lane(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
-// v of the lane CTRL / ROW_MASK selects through a DPP move (rows not in
-// ROW_MASK read 0), both 32-bit halves
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_f64(double v) {
+// The 64-bit value of the lane CTRL selects, through DPP moves of both
+// 32-bit halves; lanes outside ROW_MASK, or whose source lies outside their
+// row (row_shr), read 0. EVERY: every lane has a source (quad_perm,
+// row_ror with the full mask), so no old value is needed.
+template <int CTRL, int ROW_MASK, bool EVERY = false>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const int l = (int)(uint32_t)v, h = (int)(uint32_t)(v >> 32);
   int lo, hi;
-  if (ROW_MASK == 0xF) {  // every lane written: no old value
-    lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  if (EVERY) {
+    lo = __builtin_amdgcn_mov_dpp(l, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp(h, CTRL, 0xF, 0xF, false);
   } else {
-    lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+    lo = __builtin_amdgcn_update_dpp(0, l, CTRL, ROW_MASK, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, h, CTRL, ROW_MASK, 0xF, false);
   }
-  return __hiloint2double(hi, lo);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+template <int CTRL, int ROW_MASK, bool EVERY = false>
+__device__ __forceinline__ double dpp_f64(double v) {
+  return __longlong_as_double((long long)dpp_u64<CTRL, ROW_MASK, EVERY>((uint64_t)__double_as_longlong(v)));
 }
 
 // Sum of v over the 64 lanes of a wave, returned in every lane; the whole
@@ -112,13 +120,41 @@ __device__ __forceinline__ double dpp_f64(double v) {
 // (row_bcast 31 into rows 2 and 3); the total ends in lane 63. A fixed
 // order of additions.
 __device__ __forceinline__ double wave_sum(double v) {
-  v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
-  v += dpp_f64<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
-  v += dpp_f64<0x124, 0xF>(v);  // row_ror 4
-  v += dpp_f64<0x128, 0xF>(v);  // row_ror 8
-  v += dpp_f64<0x142, 0xA>(v);  // row_bcast 15 into rows 1 and 3
-  v += dpp_f64<0x143, 0xC>(v);  // row_bcast 31 into rows 2 and 3
+  v += dpp_f64<0xB1, 0xF, true>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E, 0xF, true>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x124, 0xF, true>(v);  // row_ror 4
+  v += dpp_f64<0x128, 0xF, true>(v);  // row_ror 8
+  v += dpp_f64<0x142, 0xA>(v);        // row_bcast 15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xC>(v);        // row_bcast 31 into rows 2 and 3
   return bcast_lane(v, 63);
+}
+
+// Inclusive prefix sum over the lanes (lane l: lanes 0 .. l, added in a
+// fixed order) by DPP, for double or int64_t; the whole wave must be active.
+// Within each row of 16 lanes row_shr 1, 2, 4, 8 (lanes shifted in from
+// outside the row add 0), then row 0's total into row 1 and row 2's into
+// row 3 (row_bcast 15), then the total of rows 0 and 1 into rows 2 and 3
+// (row_bcast 31).
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+  static_assert(sizeof(T) == 8, "64-bit scans only");
+  auto step = [](T x, uint64_t moved) {
+    T y;
+    __builtin_memcpy(&y, &moved, 8);
+    return x + y;
+  };
+  auto bits = [](T x) {
+    uint64_t u;
+    __builtin_memcpy(&u, &x, 8);
+    return u;
+  };
+  v = step(v, dpp_u64<0x111, 0xF>(bits(v)));  // row_shr 1
+  v = step(v, dpp_u64<0x112, 0xF>(bits(v)));  // row_shr 2
+  v = step(v, dpp_u64<0x114, 0xF>(bits(v)));  // row_shr 4
+  v = step(v, dpp_u64<0x118, 0xF>(bits(v)));  // row_shr 8
+  v = step(v, dpp_u64<0x142, 0xA>(bits(v)));  // row_bcast 15 into rows 1 and 3
+  v = step(v, dpp_u64<0x143, 0xC>(bits(v)));  // row_bcast 31 into rows 2 and 3
+  return v;
 }
 
 // LDS float add without return (ds_add_f64 / ds_add_f32 on gfx950).

@@ -40,12 +40,7 @@ __device__ inline void refine_wave(const double* h /* LDS, 50 */, double* __rest
   const double ave = bcast_lane(wave_sum(wei), 0) / (double)kBins;  // vflow.py:166
 
   // inclusive prefix sums of the weights (lanes >= 50 hold 0)
-  double c = wei;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double t = __shfl_up(c, o, 64);
-    if (lane >= o) c += t;
-  }
+  const double c = wave_inclusive_scan(wei);
   const double target = (double)lane * ave;  // cut k = lane
   // n = #{b < 50 : c_b < target}: c is non-decreasing over lanes 0..63 (lanes
   // >= 50 repeat the total), so binary lifting over lanes; every lane runs

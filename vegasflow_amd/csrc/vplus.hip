@@ -166,12 +166,7 @@ __device__ __forceinline__ double block_sum(double v, double* part) {
 __device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
   __shared__ int64_t wtot[kWaves];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int64_t inc = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int64_t t = __shfl_up(inc, o, 64);
-    if (lane >= o) inc += t;
-  }
+  const int64_t inc = wave_inclusive_scan(v);
   if (lane == 63) wtot[w] = inc;
   __syncthreads();
   int64_t before = 0, all = 0;
