@@ -369,7 +369,7 @@ __global__ __launch_bounds__(BS) void vegas_digest_pairs(StepK s, const FT* __re
   const uint32_t top = kBins - 1;
   double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * BS;
-  const int64_t i0 = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int64_t i0 = spread_first_event();
   if (words) {
     // kDigestU events per trip, all their loads issued before the first is
     // used: with one workgroup per CU the loop is otherwise bound by one
