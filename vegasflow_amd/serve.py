@@ -154,6 +154,10 @@ class Server:
         self.ress2 = torch.zeros(S * MAX_CUBES, dtype=torch.float64, device=self.device)
         self.outs = torch.zeros(S * OUT_LEN, dtype=torch.float64, device=self.device)
         self._linspace = torch.linspace(0, 1, BINS_MAX + 1, dtype=torch.float64, device=self.device)
+        # pinned host copies of the per-slot state a tick reads back
+        self._h_stats = torch.empty(4 * S, dtype=torch.float64, pin_memory=True)
+        self._h_draws = torch.empty(S, dtype=torch.int32, pin_memory=True)
+        self._h_next = torch.empty(S, dtype=torch.float64, pin_memory=True)
         self.free = list(range(S - 1, -1, -1))
         self.live = {}  # slot -> Request
         self.queue = collections.deque()
@@ -339,8 +343,7 @@ class Server:
 
         jobs_dev = dev_pairs(jobs) if jobs else None
         vp = (vp_fams, dev_pairs(vp_chunks), len(vp_chunks), dev_pairs(vp_dims), len(vp_dims)) if vp_chunks else None
-        slots_dev = torch.tensor(slots, dtype=torch.int64, device=self.device)
-        self._launch = (fams, jobs_dev, len(jobs), vp, slots, slots_dev)
+        self._launch = (fams, jobs_dev, len(jobs), vp, slots)
 
     def tick(self):
         """One iteration for every live slot; returns the requests retired."""
@@ -349,7 +352,7 @@ class Server:
         t_start = time.perf_counter()
         if self._launch is None:
             self._build_launch()
-        fams, jobs_dev, n_jobs, vp, slots, slots_dev = self._launch
+        fams, jobs_dev, n_jobs, vp, slots = self._launch
         for fam, tiles_dev, n_tiles, max_dim, events, pb in fams:
             if pb > 0:
                 kernels.batch_fused_pairs(self.items, self.capacity, tiles_dev, n_tiles, fam, max_dim,
@@ -369,23 +372,30 @@ class Server:
             kernels.batch_vplus_finalize(self.vp_items, chunks_dev, n_chunks, dims_dev, n_dims, BETA, self.grids,
                                          self.n_ev, self.offsets, self.ress, self.ress2, self.rows, self.outs,
                                          self.draws, self.stats)
-        # the live slots' counters and running sums only, with the VEGAS+
-        # slots' events of the next iteration (one small copy; synchronises)
-        live = torch.cat([self.stats.view(-1, 4).index_select(0, slots_dev),
-                          self.draws.index_select(0, slots_dev).to(torch.float64).unsqueeze(1),
-                          self.outs.view(-1, OUT_LEN)[:, 2].index_select(0, slots_dev).unsqueeze(1)], 1)
-        rows = live.cpu().numpy()
+        # counters and running sums of the slots up to the highest live one
+        # (slots are handed out lowest first), and the VEGAS+ slots' events of
+        # the next iteration: asynchronous copies into pinned memory, then one
+        # synchronisation
+        n = slots[-1] + 1
+        self._h_stats[:4 * n].copy_(self.stats[:4 * n], non_blocking=True)
+        self._h_draws[:n].copy_(self.draws[:n], non_blocking=True)
+        if vp is not None:
+            self._h_next[:n].copy_(self.outs.view(-1, OUT_LEN)[:n, 2], non_blocking=True)
+        torch.cuda.current_stream(self.device).synchronize()
+        stats = self._h_stats[:4 * n].numpy().reshape(n, 4)
+        draws = self._h_draws[:n].numpy()
+        nexts = self._h_next[:n].numpy()
         now = time.perf_counter()
         self.ticks += 1
         self.tick_seconds.append(now - t_start)
         retired = []
-        for r_i, s in enumerate(slots):
+        for s in slots:
             req = self.live[s]
-            st = rows[r_i]
-            k = int(st[4])
+            st = stats[s]
+            k = int(draws[s])
             if req.algorithm == "vegas+":
                 self.events_done += req._vp["events"]
-                req._vp["events"] = int(st[5])
+                req._vp["events"] = int(nexts[s])
             with np.errstate(divide="ignore", invalid="ignore"):
                 est = st[3] / st[2]
                 err = np.sqrt(1.0 / st[2])
