@@ -60,9 +60,10 @@ def _oracle_cube_sums(u, st, grid, d):
 
 def _setup(d, n, adaptive, rng, ragged=True, n_strat=None):
     st = ora.vplus_setup(d, n, adaptive)
-    if n_strat is not None:  # more cubes than the reference's 1e4 cap
+    if n_strat is not None:  # more cubes than the reference's 1e4 cap, or (n_strat, min_neval)
+        n_strat, min_neval = n_strat if isinstance(n_strat, tuple) else (n_strat, 2)
         st["n_strat"], st["n_cubes"] = n_strat, n_strat**d
-        st["min_neval"] = 2
+        st["min_neval"] = min_neval
     if ragged:
         st["n_ev"] = rng.integers(st["min_neval"], st["min_neval"] + 40, size=st["n_cubes"]).astype(np.int32)
     return st
@@ -101,11 +102,16 @@ def test_vplus_sample_bitwise(d, n):
 
 @pytest.mark.parametrize("adaptive", [True, False])
 @pytest.mark.parametrize("d,n,n_strat", [(2, 20000, None), (4, 200000, None), (5, 100000, None),
-                                         (2, 200000, 150), (1, 30000, None), (15, 40000, None)])
+                                         (2, 200000, 150), (1, 30000, None), (15, 40000, None),
+                                         (3, 400000, (8, 350))])
 def test_vplus_fused_iteration_matches_oracle(d, n, n_strat, adaptive):
     """n_strat = 150 at d = 2 gives 22500 cubes, past the reference's cap of
     1e4 (the engine takes up to 2^24 - 1). d = 2..14 train through pair
-    tables, d = 1 and 15 through per-dimension histograms."""
+    tables, d = 1 and 15 through per-dimension histograms. Cube sizes cover
+    every path of the run loop: cubes of a few events (64-event trips across
+    many cubes), of 20 to 60 (trips across one boundary), and of several
+    hundred (n_strat = 8 at d = 3, 350 to 389 events per cube: trips inside
+    one cube), and one cube of all events at d = 15."""
     rng = np.random.default_rng(10 * d + adaptive)
     st = _setup(d, n, adaptive, rng, n_strat=n_strat)
     grid = random_grid(rng, d)
