@@ -160,7 +160,14 @@ def bench_unfused(vf, dtype, steps=10):
     res, sig = inst.run_integration(5)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return {"events_per_s_run_integration": 5 * N / dt, "integral_5_iterations": float(res),
+    # the whole iteration's algorithmic bytes: x written and read, w and the
+    # bins written and read, f written and read (2 s d + 4 s + 2 d per event)
+    it_bytes = 2 * s * D + 4 * s + 2 * D
+    return {"events_per_s_run_integration": 5 * N / dt,
+            "iteration_bytes_per_event": it_bytes,
+            "iteration_GB_s": it_bytes * 5 * N / dt / 1e9,
+            "iteration_frac_of_hbm_peak": it_bytes * 5 * N / dt / 8e12,
+            "integral_5_iterations": float(res),
             "sigma": float(sig),
             "sample_kernel": {"ms": ms[0], "bytes_per_event": b["sample"],
                               "achieved_GB_s": b["sample"] * N / (ms[0] * 1e-3) / 1e9},
