@@ -1,8 +1,7 @@
-// batch_vplus.hpp - the producer of batched VegasFlowPlus integrals
+// batch_vplus.hip - the producer of batched VegasFlowPlus integrals
 // (mv_batch_vplus_fused): many VEGAS+ integrals advance one iteration in one
 // persistent launch (vflowplus.py:37-80 + 187-220 per item). The iteration
-// end of the items is in vplus.hip (mv_batch_vplus_finalize);
-// batch_vplus_mixed.hip and batch_vplus_f64.hip compile one precision each.
+// end of the items is in vplus.hip (mv_batch_vplus_finalize).
 //
 // Workgroups stride over tiles of consecutive events of one item. Per tile:
 // the item's map table (P, Q, W = dx * 50) in LDS, histograms zeroed, then
@@ -14,8 +13,6 @@
 // upper bound of the adaptive event count) does nothing. Launches whose
 // items share n_dim run a kernel compiled for it (the registers of one event
 // body); mixed dimensions choose the body per tile.
-#pragma once
-
 #include <algorithm>
 #include <array>
 #include <type_traits>
@@ -217,6 +214,22 @@ int launch_batch_vplus_precision(const mv_batch_vplus_item* items, const mv_batc
   hipLaunchKernelGGL(k, dim3(blocks), dim3(bs), lds, stream, items, tiles, n_tiles, draws, grids, n_ev, offsets,
                      ress, ress2, rows, nbmax);
   return check_launch("mv_batch_vplus_fused");
+}
+
+int launch_batch_vplus_mixed(const mv_batch_vplus_item* items, const mv_batch_tile* tiles, int n_tiles, int family,
+                             int min_dim, int max_dim, const int32_t* draws, const double* grids,
+                             const int32_t* n_ev, const int32_t* offsets, double* ress, double* ress2, double* rows,
+                             int blocks, hipStream_t stream) {
+  return launch_batch_vplus_precision<true>(items, tiles, n_tiles, family, min_dim, max_dim, draws, grids, n_ev,
+                                            offsets, ress, ress2, rows, blocks, stream);
+}
+
+int launch_batch_vplus_f64(const mv_batch_vplus_item* items, const mv_batch_tile* tiles, int n_tiles, int family,
+                           int min_dim, int max_dim, const int32_t* draws, const double* grids,
+                           const int32_t* n_ev, const int32_t* offsets, double* ress, double* ress2, double* rows,
+                           int blocks, hipStream_t stream) {
+  return launch_batch_vplus_precision<false>(items, tiles, n_tiles, family, min_dim, max_dim, draws, grids, n_ev,
+                                             offsets, ress, ress2, rows, blocks, stream);
 }
 
 }  // namespace mv

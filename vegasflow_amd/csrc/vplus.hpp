@@ -129,10 +129,10 @@ __device__ __forceinline__ void segment_add(int cube, double v, double v2, doubl
 
 
 
-// The fused VEGAS+ producer (vplus_fused.hpp), compiled per (integrand
+// The fused VEGAS+ producer (vplus_fused.hip), compiled per (integrand
 // family, train, n_dim, injected draws) for 1 <= n_dim <= MV_FUSED_MAX_DIM,
-// one precision per source file. Return the workgroups launched or a
-// negative error.
+// one launcher per precision. Return the workgroups launched or a negative
+// error.
 int launch_vplus_fused_mixed(int family, bool train, const StepK& k, const StrataK& st,
                              const mv_integrand& fn, const double* grid, double* ress, double* ress2,
                              const RowsK& R, hipStream_t s);
@@ -140,15 +140,15 @@ int launch_vplus_fused_f64(int family, bool train, const StepK& k, const StrataK
                            const mv_integrand& fn, const double* grid, double* ress, double* ress2,
                            const RowsK& R, hipStream_t s);
 
-// The batched VEGAS+ producer (batch_vplus.hpp) of one precision: 0 or a
+// The batched VEGAS+ producer (batch_vplus.hip) of one precision: 0 or a
 // negative status.
 int launch_batch_vplus_mixed(const mv_batch_vplus_item* items, const mv_batch_tile* tiles, int n_tiles, int family,
                              int min_dim, int max_dim, const int32_t* draws, const double* grids,
                              const int32_t* n_ev, const int32_t* offsets, double* ress, double* ress2, double* rows,
                              int blocks, hipStream_t stream);
 int launch_batch_vplus_f64(const mv_batch_vplus_item* items, const mv_batch_tile* tiles, int n_tiles, int family,
-                             int min_dim, int max_dim, const int32_t* draws, const double* grids,
-                             const int32_t* n_ev, const int32_t* offsets, double* ress, double* ress2, double* rows,
-                             int blocks, hipStream_t stream);
+                           int min_dim, int max_dim, const int32_t* draws, const double* grids,
+                           const int32_t* n_ev, const int32_t* offsets, double* ress, double* ress2, double* rows,
+                           int blocks, hipStream_t stream);
 
 }  // namespace mv
