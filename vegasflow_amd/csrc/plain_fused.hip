@@ -14,22 +14,21 @@
 
 namespace mv {
 
-// INJ: injected draws (s.inject, parity tests) instead of Philox, a
-// compile-time choice so that the event body has no test per dimension.
-template <int FAM, bool MIXED, int D, bool INJ>
-__global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, RowsK R) {
-  const uint32_t draw = draw_of(s);
-  using T = typename std::conditional<MIXED, float, double>::type;
+// The events of this thread (plain.py:18-35): uniform points, with the
+// integration limits (LIM, monte_carlo.py:271-274) or without, the
+// integrand, and the sums of w f and (w f)^2. INJ: injected draws
+// (s.inject, parity tests) instead of Philox. Both are compile-time choices,
+// so that the event body has no test per dimension.
+template <int FAM, bool MIXED, int D, bool INJ, bool LIM, typename T>
+__device__ __forceinline__ void plain_events(const StepK& s, uint32_t draw, Integrand<T, FAM>& F, double wt,
+                                             double& acc, double& acc2) {
   const int d = D ? D : s.n_dim;
   const int groups = D ? (D + 3) / 4 : s.groups;
-  Integrand<T, FAM> F(fn);
-  const bool limits = s.xmin != nullptr;
-  const double wt = limits ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
-  double acc = 0.0, acc2 = 0.0;
   const int64_t step = (int64_t)gridDim.x * kBlock;
   for (int64_t i = spread_first_event(); i < s.n_events; i += step) {
     const uint32_t c0 = s.ev_lo + (uint32_t)i, c1 = s.ev_hi;  // launch-uniform high word
     F.begin();
+#pragma unroll
     for (int g = 0; g < groups; ++g) {
       uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
       if (!INJ) {
@@ -44,22 +43,35 @@ __global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, 
           if (MIXED) {
             const uint32_t hw = INJ ? static_cast<const uint32_t*>(s.inject)[i * d + j] : word(h, q);
             float xf = fmaf((float)(hw >> 9), 0x1p-23f, 0x1p-24f);
-            if (limits) xf = fmaf(xf, (float)s.xdelta[j], (float)s.xmin[j]);
+            if (LIM) xf = fmaf(xf, (float)s.xdelta[j], (float)s.xmin[j]);
             x = (T)xf;
           } else {
             const double u = INJ ? static_cast<const double*>(s.inject)[i * d + j]
                                  : draw_f64(word(h, q), word(l, q));
-            x = (T)(limits ? s.xmin[j] + u * s.xdelta[j] : u);
+            x = (T)(LIM ? s.xmin[j] + u * s.xdelta[j] : u);
           }
           F.add(j, x);
         }
       }
     }
     const double tmp = (double)F.value() * wt;  // plain.py:27
-    const double tmp2 = tmp * tmp;
     acc += tmp;
-    acc2 += tmp2;
+    acc2 += tmp * tmp;
   }
+}
+
+template <int FAM, bool MIXED, int D, bool INJ>
+__global__ __launch_bounds__(kBlock) void plain_fused(StepK s, mv_integrand fn, RowsK R) {
+  const uint32_t draw = draw_of(s);
+  using T = typename std::conditional<MIXED, float, double>::type;
+  Integrand<T, FAM> F(fn);
+  const bool limits = s.xmin != nullptr;
+  const double wt = limits ? s.xjac * s.xdeltajac : s.xjac;  // monte_carlo.py:270-274
+  double acc = 0.0, acc2 = 0.0;
+  if (limits)
+    plain_events<FAM, MIXED, D, INJ, true>(s, draw, F, wt, acc, acc2);
+  else
+    plain_events<FAM, MIXED, D, INJ, false>(s, draw, F, wt, acc, acc2);
   finish_block<false>(acc, acc2, nullptr, 0, R);
 }
 
