@@ -429,6 +429,75 @@ def bench_issue_costs(iters=4096):
     return out
 
 
+LDS_KINDS = ("ds_read_b64, random map entries", "ds_read_b64, lane-consecutive entries",
+             "ds_add_f64, random pair cells", "ds_add_f64, lane-consecutive cells",
+             "8 random reads + 4 random adds (mean)")
+
+
+def bench_lds_costs(iters=8192):
+    """Shader cycles per LDS wave instruction per CU (libmvprobe.so) at the
+    d = 8 pair kernel's launch shape: one 1024-thread workgroup per CU, 83.2
+    KB of LDS laid out as its map table and pair cells."""
+    import ctypes
+
+    from vegasflow_amd import _build
+
+    lib = ctypes.CDLL(_build.probe_path())
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.mvp_lds_cost.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp]
+    lib.mvp_lds_cost.restype = ctypes.c_int
+    out = {}
+    for k in range(lib.mvp_lds_kinds()):
+        ms, ghz, per = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        if lib.mvp_lds_cost(k, iters, ctypes.byref(ms), ctypes.byref(ghz), ctypes.byref(per)) != 0:
+            raise RuntimeError(f"LDS probe {LDS_KINDS[k]} failed")
+        out[LDS_KINDS[k]] = round(ms.value * 1e-3 * ghz.value * 1e9 / per.value, 2)
+    return out
+
+
+def bench_scaling(torch, clock_hz=CLOCK_HZ, sizes=(2_500_000, 5_000_000, 10_000_000, 20_000_000), reps=20):
+    """Fused training kernel time against events per launch, t(N) = a + b N
+    (HIP events around each launch, the GPU kept busy while the host
+    enqueues it): a is the launch's fixed cost (launch, table build, LDS
+    clearing, epilogue), b the event loop's; the loop's VALU-busy fraction
+    is the counted busy cycles per 64 events over b in cycles per wave64
+    iteration per SIMD."""
+    import numpy as np
+
+    from vegasflow_amd import integrands, kernels
+
+    dev = torch.device("cuda", 0)
+    fn = integrands.Gaussian(A, 0.5).to_struct(D)
+    simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+    out = {}
+    for prec in ("mixed", "f64"):
+        ws = kernels.Workspace(dev, kernels.partial_stride(D, 1, True))
+        grid = torch.linspace(0, 1, 51, dtype=torch.float64, device=dev).repeat(D, 1).contiguous()
+        points = []
+        for n in sizes:
+            evs = []
+            for k in range(reps + 3):
+                step = kernels.make_step(D, n, precision=prec, train=True, seed=3, draw_id=k, xjac=1.0 / n)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda._sleep(1_000_000)  # the host's enqueue latency out of the timed region
+                e0.record()
+                rows = kernels.vegas_fused(step, fn, grid, ws)
+                e1.record()
+                kernels.vegas_finalize(D, 1, True, False, n, ws, rows)  # consume the rows
+                if k >= 3:
+                    evs.append((e0, e1))
+            torch.cuda.synchronize()
+            points.append((n, float(np.median([a.elapsed_time(b) for a, b in evs]))))
+        ns = np.array([p[0] for p in points], float)
+        ts = np.array([p[1] for p in points], float) * 1e-3
+        b, a = np.polyfit(ns, ts, 1)
+        cycles = b * clock_hz * 64 * simds  # per wave64 iteration (64 events) per SIMD
+        out[prec] = {"points_ms": points, "fixed_us": a * 1e6, "us_per_1e7_events": b * 1e13,
+                     "cycles_per_64_events_per_simd": cycles,
+                     "loop_valu_busy_frac": VALU_BUSY_CYCLES[prec] / cycles, "clock_GHz": clock_hz / 1e9}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,7 +510,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the VEGAS+, batched and serving runs")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the Poisson stream (60 s: the serving configuration)")
-    ap.add_argument("--only", choices=["unfused", "plain", "vplus", "batch", "serve", "issue"],
+    ap.add_argument("--only", choices=["unfused", "plain", "vplus", "batch", "serve", "issue", "scaling"],
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -469,7 +538,9 @@ def main():
         elif args.only == "batch":
             out = {p: bench_batched(torch, dt_) for p, dt_ in precisions}
         elif args.only == "issue":
-            out = bench_issue_costs()
+            out = {"valu": bench_issue_costs(), "lds_per_cu": bench_lds_costs()}
+        elif args.only == "scaling":
+            out = bench_scaling(torch)
         else:
             out = {p: bench_serving(args.serve_seconds, dt_) for p, dt_ in precisions}
         print(json.dumps({"section": args.only, "result": out}))
@@ -556,8 +627,12 @@ def main():
     line["roofline"]["f64_kernel"]["frac"] = line["roofline"]["f64_kernel"]["achieved"] / PEAK["f64"]
     try:  # the issue costs the busy cycles are priced at, measured on this GPU now
         line["roofline"]["valu_issue_cycles"] = bench_issue_costs()
+        line["roofline"]["lds_cycles_per_cu"] = bench_lds_costs()
     except (OSError, RuntimeError) as e:
         line["roofline"]["valu_issue_cycles"] = {"error": str(e)}
+    # the event loop apart from the launch's fixed cost (at the 2.4 GHz clock
+    # the kernel runs at: GRBM_GUI_ACTIVE over its traced duration)
+    line["roofline"]["loop"] = bench_scaling(torch)
     if not args.no_extra:
         line["unfused_torch_integrand"] = {p: bench_unfused(vf, dt_) for p, dt_ in
                                            (("mixed", torch.float32), ("f64", torch.float64))}

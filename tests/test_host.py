@@ -356,3 +356,6 @@ def test_issue_cost_probe_library_loads():
     d = ctypes.c_double()
     assert probe.mvp_issue_cost(-1, 4096, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == -1
     assert probe.mvp_issue_cost(probe.mvp_kinds(), 4096, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == -1
+    assert probe.mvp_lds_kinds() == len(bench.LDS_KINDS)
+    assert probe.mvp_lds_cost(-1, 4096, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == -1
+    assert probe.mvp_lds_cost(probe.mvp_lds_kinds(), 4096, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == -1
