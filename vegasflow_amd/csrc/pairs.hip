@@ -6,9 +6,9 @@
 // the per-group and per-dimension tests of a runtime n_dim, and the branches
 // around them, disappear (measured 68.5 -> 56.3 us at d = 8, 1e7 events).
 // Histograms are pair tables (fused_event.hpp): dimensions 2p and 2p + 1
-// share one 50 x 50 table, one LDS add per pair of dimensions. LDS: the map
-// table (n_dim * 50 float2), then pair_cells(n_dim) doubles, about 83 KB at
-// d = 8, so one kPairBlock-thread workgroup per CU.
+// share one 50 x 50 table, one LDS add per pair of dimensions. LDS (static):
+// the map table (n_dim * 50 float2), then pair_cells(n_dim) doubles, about
+// 83 KB at d = 8, so one kPairBlock-thread workgroup per CU.
 #include "abi_util.hpp"
 #include "device.hpp"
 #include "fused_event.hpp"
@@ -21,8 +21,11 @@ __global__ __launch_bounds__(kPairBlock) void vegas_fused_pairs(StepK s, mv_inte
                                                                 const double* __restrict__ grid,
                                                                 RowsK R) {
   const uint32_t draw = draw_of(s);
-  extern __shared__ double smem[];
+  // Static LDS (its size is a compile-time function of D): its addresses are
+  // constants that the LDS instructions take as offsets, where the base of
+  // dynamic LDS would cost an add per table address.
   constexpr int nb = D * kBins;
+  __shared__ __align__(16) double smem[nb + pair_cells(D)];
   float2* tab = reinterpret_cast<float2*>(smem);
   double* H = smem + nb;  // 16-byte aligned: nb * 8 is a multiple of 400
   build_table<float, FAM>(fn, grid, D, s.xmin, s.xdelta, tab, nullptr);
@@ -58,28 +61,26 @@ __global__ __launch_bounds__(kPairBlock) void vegas_fused_pairs(StepK s, mv_inte
 namespace {
 
 template <int FAM, int D>
-int launch_d(const StepK& k, const mv_integrand& fn, const double* grid, const RowsK& R, size_t lds,
-             hipStream_t st) {
-  return launch_producer<kPairBlock>(vegas_fused_pairs<FAM, D>, k.n_events, lds, st, k, fn, grid, R);
+int launch_d(const StepK& k, const mv_integrand& fn, const double* grid, const RowsK& R, hipStream_t st) {
+  return launch_producer<kPairBlock>(vegas_fused_pairs<FAM, D>, k.n_events, 0, st, k, fn, grid, R);
 }
 
 template <int FAM>
-int launch_fam(const StepK& k, const mv_integrand& fn, const double* grid, const RowsK& R, size_t lds,
-               hipStream_t st) {
+int launch_fam(const StepK& k, const mv_integrand& fn, const double* grid, const RowsK& R, hipStream_t st) {
   switch (k.n_dim) {
-    case 2: return launch_d<FAM, 2>(k, fn, grid, R, lds, st);
-    case 3: return launch_d<FAM, 3>(k, fn, grid, R, lds, st);
-    case 4: return launch_d<FAM, 4>(k, fn, grid, R, lds, st);
-    case 5: return launch_d<FAM, 5>(k, fn, grid, R, lds, st);
-    case 6: return launch_d<FAM, 6>(k, fn, grid, R, lds, st);
-    case 7: return launch_d<FAM, 7>(k, fn, grid, R, lds, st);
-    case 8: return launch_d<FAM, 8>(k, fn, grid, R, lds, st);
-    case 9: return launch_d<FAM, 9>(k, fn, grid, R, lds, st);
-    case 10: return launch_d<FAM, 10>(k, fn, grid, R, lds, st);
-    case 11: return launch_d<FAM, 11>(k, fn, grid, R, lds, st);
-    case 12: return launch_d<FAM, 12>(k, fn, grid, R, lds, st);
-    case 13: return launch_d<FAM, 13>(k, fn, grid, R, lds, st);
-    case 14: return launch_d<FAM, 14>(k, fn, grid, R, lds, st);
+    case 2: return launch_d<FAM, 2>(k, fn, grid, R, st);
+    case 3: return launch_d<FAM, 3>(k, fn, grid, R, st);
+    case 4: return launch_d<FAM, 4>(k, fn, grid, R, st);
+    case 5: return launch_d<FAM, 5>(k, fn, grid, R, st);
+    case 6: return launch_d<FAM, 6>(k, fn, grid, R, st);
+    case 7: return launch_d<FAM, 7>(k, fn, grid, R, st);
+    case 8: return launch_d<FAM, 8>(k, fn, grid, R, st);
+    case 9: return launch_d<FAM, 9>(k, fn, grid, R, st);
+    case 10: return launch_d<FAM, 10>(k, fn, grid, R, st);
+    case 11: return launch_d<FAM, 11>(k, fn, grid, R, st);
+    case 12: return launch_d<FAM, 12>(k, fn, grid, R, st);
+    case 13: return launch_d<FAM, 13>(k, fn, grid, R, st);
+    case 14: return launch_d<FAM, 14>(k, fn, grid, R, st);
     default: return fail(MV_ERR_UNSUPPORTED, "pair kernel: n_dim %d outside %d..%d", k.n_dim,
                          kPairMinDim, kPairMaxDim);
   }
@@ -89,13 +90,13 @@ static_assert(kPairMinDim == 2 && kPairMaxDim == 14, "launch_fam covers n_dim 2.
 }  // namespace
 
 int launch_vegas_fused_pairs(int family, const StepK& k, const mv_integrand& fn, const double* grid,
-                             const RowsK& R, size_t lds, hipStream_t st) {
+                             const RowsK& R, hipStream_t st) {
   switch (family) {
-    case MV_FN_CONSTANT: return launch_fam<MV_FN_CONSTANT>(k, fn, grid, R, lds, st);
-    case MV_FN_GAUSSIAN: return launch_fam<MV_FN_GAUSSIAN>(k, fn, grid, R, lds, st);
-    case MV_FN_PRODUCT: return launch_fam<MV_FN_PRODUCT>(k, fn, grid, R, lds, st);
-    case MV_FN_GAUSSMIX: return launch_fam<MV_FN_GAUSSMIX>(k, fn, grid, R, lds, st);
-    case MV_FN_BWPOWER: return launch_fam<MV_FN_BWPOWER>(k, fn, grid, R, lds, st);
+    case MV_FN_CONSTANT: return launch_fam<MV_FN_CONSTANT>(k, fn, grid, R, st);
+    case MV_FN_GAUSSIAN: return launch_fam<MV_FN_GAUSSIAN>(k, fn, grid, R, st);
+    case MV_FN_PRODUCT: return launch_fam<MV_FN_PRODUCT>(k, fn, grid, R, st);
+    case MV_FN_GAUSSMIX: return launch_fam<MV_FN_GAUSSMIX>(k, fn, grid, R, st);
+    case MV_FN_BWPOWER: return launch_fam<MV_FN_BWPOWER>(k, fn, grid, R, st);
     default: return fail(MV_ERR_UNSUPPORTED, "pair kernel: unknown family %d", family);
   }
 }

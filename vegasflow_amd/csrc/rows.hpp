@@ -238,11 +238,11 @@ int launch_plain_fused_dims(int family, bool mixed, const StepK& k, const mv_int
                             const RowsK& R, hipStream_t st);
 
 // The mixed-precision training producer with pair histograms (pairs.hip):
-// kPairBlock-thread workgroups, ``lds`` bytes of table and pair cells, one
-// kernel per (family, n_dim) for kPairMinDim <= n_dim <= kPairMaxDim.
-// Returns the workgroups launched.
+// kPairBlock-thread workgroups, n_dim * 50 float2 table entries and
+// pair_cells(n_dim) doubles of static LDS, one kernel per (family, n_dim)
+// for kPairMinDim <= n_dim <= kPairMaxDim. Returns the workgroups launched.
 int launch_vegas_fused_pairs(int family, const StepK& k, const mv_integrand& fn, const double* grid,
-                             const RowsK& R, size_t lds, hipStream_t st);
+                             const RowsK& R, hipStream_t st);
 
 // The FP64 training producer with pair histograms (pairs_f64.hip), as
 // launch_vegas_fused_pairs; ``lds`` = fused_pairs_f64_lds(n_dim).

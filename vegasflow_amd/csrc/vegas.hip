@@ -525,9 +525,9 @@ extern "C" int32_t mv_producer_rows(int64_t n_events) {
 
 namespace {
 
-// LDS bytes of vegas_fused_pairs, 0 where it does not apply (FP64, no
-// training, n_dim outside kPairMinDim..kPairMaxDim, or more LDS than a
-// workgroup may have)
+// LDS bytes of vegas_fused_pairs' table and cells (static in the kernel),
+// 0 where it does not apply (FP64, no training, n_dim outside
+// kPairMinDim..kPairMaxDim, or more LDS than a workgroup may have)
 size_t pairs_lds(int d, bool mixed, bool train) {
   if (!mixed || !train || d < kPairMinDim || d > kPairMaxDim) return 0;
   const size_t lds = (size_t)(d * kBins + pair_cells(d)) * sizeof(double);
@@ -537,8 +537,7 @@ size_t pairs_lds(int d, bool mixed, bool train) {
 template <int FAM>
 int launch_vegas_fused(const StepK& k, const mv_integrand& fn, bool mixed, bool train,
                        const double* grid, const RowsK& R, hipStream_t st) {
-  if (const size_t plds = pairs_lds(k.n_dim, mixed, train))
-    return launch_vegas_fused_pairs(FAM, k, fn, grid, R, plds, st);
+  if (pairs_lds(k.n_dim, mixed, train) != 0) return launch_vegas_fused_pairs(FAM, k, fn, grid, R, st);
   if (!mixed && train && k.n_dim >= kPairMinDim && k.n_dim <= kPairMaxDim &&
       fused_pairs_f64_lds(k.n_dim) <= max_lds_per_block())
     return launch_vegas_fused_pairs_f64(FAM, k, fn, grid, R, fused_pairs_f64_lds(k.n_dim), st);
