@@ -42,8 +42,8 @@ TRAFFIC_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_pmc.md"
 # `bench.py --only issue` measures on gfx950 (2 cycles per wave64 FP32 add,
 # multiply or FMA, 8 per v_exp_f32, 4 for every other kind), set against
 # 4 SIMDs per CU at the 2.4 GHz engine clock (MI355X datasheet)
-VALU_INSTS = {"mixed": 2.535e7, "f64": 5.093e7}
-VALU_BUSY_CYCLES = {"mixed": 601.2, "f64": 1303.8}
+VALU_INSTS = {"mixed": 2.502e7, "f64": 5.107e7}
+VALU_BUSY_CYCLES = {"mixed": 588.6, "f64": 1307.3}
 VALU_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_valu_mix.md"
 CLOCK_HZ = 2.4e9
 # host cores one GPU's share of a GPU box holds (worker pools are sized to it)
@@ -67,14 +67,13 @@ def timed_steps(vf, inst, steps, warmup):
     stream = torch.cuda.current_stream()
     ws = inst._workspace_for(kernels.partial_stride(inst.n_dim, 1, True))
 
-    def one(ev=None):
+    def producer():
         inst._next_draw()
         step = inst._step(inst.n_events, train=True, offset=0)
-        if ev is not None:
-            ev[0].record(stream)
-        ng = kernels.vegas_fused(step, inst._fused, inst.divisions, ws)
-        if ev is not None:
-            ev[1].record(stream)
+        return kernels.vegas_fused(step, inst._fused, inst.divisions, ws)
+
+    def one():
+        ng = producer()
         return kernels.vegas_finalize(inst.n_dim, 1, True, True, inst.n_events, ws, ng, grid=inst.divisions)
 
     for _ in range(warmup):
@@ -87,12 +86,20 @@ def timed_steps(vf, inst, steps, warmup):
         out = one()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    # kernel time: K more steps with HIP events around the fused kernel
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-    for k in range(steps):
-        one(evs[k])
+    # kernel time: K producer launches back to back between one pair of HIP
+    # events on the launch stream (the pair's own cost spread over K; a spin
+    # kernel first lets the host enqueue ahead of the GPU), then one finalize
+    # that consumes the summed rows without refining
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda._sleep(5_000_000)
+    e0.record(stream)
+    rows = 1
+    for _ in range(steps):
+        rows = producer()
+    e1.record(stream)
+    kernels.vegas_finalize(inst.n_dim, 1, True, False, inst.n_events, ws, rows)
     torch.cuda.synchronize()
-    kms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    kms = e0.elapsed_time(e1) / steps
     return dt, kms, out
 
 
