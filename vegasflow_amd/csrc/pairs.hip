@@ -48,9 +48,24 @@ __global__ __launch_bounds__(kPairBlock) void vegas_fused_pairs(StepK s, mv_inte
       acc2 += tmp * tmp;
     }
   } else {
-    for (int64_t i = i0; i < s.n_events; i += step) {
-      const double tmp = event_mixed_pairs<FAM, false>(F, s.ev_lo + (uint32_t)i, s.ev_hi, draw,
-                                                       s.key, D, tab, cs, H, nullptr, wscale);
+    // The wave's trips in which all 64 lanes have an event run without a
+    // per-lane test, counted on the scalar unit; at most one trip with a
+    // per-lane test follows. The Philox counter word advances by the step
+    // (mod 2^32) as the 64-bit event index would.
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t w0 = ((int64_t)wave * gridDim.x + blockIdx.x) * 64;  // the wave's first event (i0 - lane)
+    const int64_t full = s.n_events - 64 - w0 >= 0 ? (s.n_events - 64 - w0) / step + 1 : 0;
+    uint32_t c = s.ev_lo + (uint32_t)i0;
+    for (int64_t t = 0; t < full; ++t) {
+      const double tmp = event_mixed_pairs<FAM, false>(F, c, s.ev_hi, draw, s.key, D, tab, cs, H, nullptr,
+                                                       wscale);
+      acc += tmp;
+      acc2 += tmp * tmp;
+      c += (uint32_t)step;
+    }
+    if (i0 + full * step < s.n_events) {
+      const double tmp = event_mixed_pairs<FAM, false>(F, c, s.ev_hi, draw, s.key, D, tab, cs, H, nullptr,
+                                                       wscale);
       acc += tmp;
       acc2 += tmp * tmp;
     }
