@@ -46,7 +46,7 @@ VALU_INSTS = {"mixed": 2.473e7, "f64": 5.107e7}
 VALU_BUSY_CYCLES = {"mixed": 581.2, "f64": 1307.3}
 VALU_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_valu_mix.md"
 CLOCK_HZ = 2.4e9
-# host cores one GPU's share of a GPU box holds (worker pools are sized to it)
+# host cores allotted to one GPU of the node (CPU worker pools are sized to it)
 CPU_SHARE = 16
 
 
@@ -337,8 +337,8 @@ def cpu_baseline(numpy_seconds=6.0):
     from oracle import torch_cpu as tc
     from oracle import vegas as ora
 
-    # the host cores this process may run on, capped at the CPU share a GPU
-    # box gives one GPU (16 cores; the box's process guard sizes pools to it)
+    # the host cores this process may run on, capped at the host cores
+    # allotted to one GPU (16)
     affinity = len(os.sched_getaffinity(0))
     threads = max(1, min(CPU_SHARE, affinity))
     torch.set_num_threads(threads)
