@@ -435,6 +435,23 @@ def test_mixed_fused_tracks_f64_on_same_draws():
     assert rel(h32.sum(), h64.sum()) < 1e-4
 
 
+@pytest.mark.parametrize("d,n", [(8, 37), (8, 64), (5, 262145), (8, 600037), (14, 524288 + 70)])
+def test_mixed_pair_kernel_counts_every_event_once(d, n):
+    """The pair kernel's event loop (trips in which a whole wave has events,
+    then one trip tested per lane) takes every event of the step exactly
+    once: per-bin counts of a constant integrand equal the oracle's integer
+    bins for fewer events than a wave, exactly one wave, one event more than
+    a full round of the grid (262144 threads), and several rounds with a
+    partial wave at the end."""
+    out, hist, _ = _fused_once("const", d, n, ora.initial_grid(d), True, seed=9, precision="mixed")
+    bins, _ = ora.mixed_position(ora.hi_words(9, 0, n, d))
+    tmp2 = (2.5 / n) ** 2
+    assert abs(out[0] - 2.5) < 1e-5
+    for j in range(d):
+        counts = np.bincount(bins[:, j].astype(np.int64), minlength=50)
+        np.testing.assert_allclose(hist[j] / tmp2, counts, rtol=1e-6)
+
+
 @pytest.mark.parametrize("d", [2, 3, 11, 15])
 def test_mixed_fused_bins_match_integer_oracle(d):
     """Mixed bins are those of oracle.mixed_position: compare the histogram
