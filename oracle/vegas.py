@@ -70,18 +70,17 @@ def hi_words(seed, draw_id, n, n_dim, event_offset=0):
     return words[:, :n_dim].copy()
 
 
-MIXED_P0 = 50 * 2 ** 23 - 25
-
-
 def mixed_position(words):
     """Bin and fraction of the mixed-precision path (the engine's own form;
-    the reference has only FP64/FP32 uniforms): the leading 23 bits
-    s = word >> 9 give xn = 50 (1 - (s + 1/2) / 2^23), held exactly as
-    p = xn 2^23 = 50 2^23 - 25 - 50 s. Returns (bin = p >> 23,
-    frac = (p mod 2^23) / 2^23), i.e. xn = bin + frac exactly."""
-    s = np.asarray(words).astype(np.int64) >> 9
-    p = MIXED_P0 - 50 * s
-    return (p >> 23).astype(np.int32), (p & (2 ** 23 - 1)).astype(np.float64) * 2.0 ** -23
+    the reference has only FP64/FP32 uniforms). With u taken as
+    (word + 1/2) / 2^32, the mirrored coordinate m = 50 - xn = 50 u is held
+    to 23 fraction bits as q = floor((50 word + 25) / 2^9); the point is
+    xn = 50 - q / 2^23 exactly. Returns (bin = 49 - (q >> 23),
+    frac = 1 - (q mod 2^23) / 2^23), i.e. xn = bin + frac exactly, with
+    frac in (0, 1] (frac = 1: the bin's upper edge)."""
+    q = (np.asarray(words).astype(np.int64) * 50 + 25) >> 9
+    mfrac = (q & (2 ** 23 - 1)).astype(np.float64) * 2.0 ** -23
+    return (49 - (q >> 23)).astype(np.int32), 1.0 - mfrac
 
 
 def mixed_map(words, divisions):

@@ -277,21 +277,28 @@ def test_torch_cpu_baseline_matches_the_oracle_on_the_same_draws():
 
 
 def test_mixed_position_exact_and_tracks_the_f64_draws():
-    """The mixed path's (bin, frac) is xn = 50 (1 - (s + 1/2) / 2^23) exactly
-    (rational brute force), never on a bin edge, and lands in the bin of the
+    """The mixed path's (bin, frac) is xn = 50 - floor((50 w + 25) / 2^9) /
+    2^23 exactly (rational brute force), inside (0, 50] with frac in (0, 1],
+    within 2^-23 of 50 (1 - (w + 1/2) / 2^32), and lands in the bin of the
     FP64 draw of the same word but for points within ~2^-23 of an edge."""
     from fractions import Fraction
 
     words = ora.hi_words(9, 2, 4000, 3).reshape(-1)
-    extreme = np.array([0, 1, 511, 512, 0x7FFFFFFF, 0xFFFFFDFF, 0xFFFFFFFF], dtype=np.uint32)
+    extreme = np.array([0, 1, 9, 10, 511, 512, 0x7FFFFFFF, 0xFFFFFDFF, 0xFFFFFFFF], dtype=np.uint32)
     words = np.concatenate([words, extreme])
     ind, frac = ora.mixed_position(words)
     for k in list(range(0, words.size, 37)) + list(range(words.size - extreme.size, words.size)):
-        s = int(words[k]) >> 9
-        xn = 50 * (1 - Fraction(2 * s + 1, 2 ** 24))
-        assert ind[k] == int(xn) and Fraction(float(frac[k])) == xn - int(xn)
+        w = int(words[k])
+        q = (50 * w + 25) // 512
+        xn = 50 - Fraction(q, 2 ** 23)
+        assert ind[k] == 49 - q // 2 ** 23 and Fraction(float(frac[k])) == xn - ind[k]
+        assert abs(xn - 50 * (1 - Fraction(2 * w + 1, 2 ** 33))) < Fraction(1, 2 ** 23)
     assert ind.min() >= 0 and ind.max() <= 49
-    assert frac.min() > 0 and frac.max() < 1
+    assert frac.min() > 0 and frac.max() <= 1
+    # word 0 sits on the top edge of bin 49 (x = 1); the largest word just
+    # above 0
+    i0, f0 = ora.mixed_position(np.array([0, 0xFFFFFFFF], dtype=np.uint32))
+    assert (i0[0], f0[0]) == (49, 1.0) and i0[1] == 0 and 0 < f0[1] <= 2.0 ** -22
     n, d = 20000, 4
     _, _, ind64 = ora.vegas_map(ora.uniform_f64(5, 1, n, d), ora.initial_grid(d))
     ind32, _ = ora.mixed_position(ora.hi_words(5, 1, n, d))

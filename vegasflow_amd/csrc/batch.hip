@@ -172,7 +172,7 @@ __device__ __forceinline__ void batch_tile_pairs(const mv_batch_item& it, const 
       }
     }
   }
-  finish_pairs<BS>(acc, acc2, H, it.train ? d : 0, rows + it.row_offset);
+  finish_pairs<BS, true, MIXED>(acc, acc2, H, it.train ? d : 0, rows + it.row_offset);
 }
 
 template <int FAM, int BS, bool MIXED>

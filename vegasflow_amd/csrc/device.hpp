@@ -67,17 +67,19 @@ __device__ __forceinline__ double draw_f64(uint32_t hi, uint32_t lo) {
 }
 
 // Mixed-precision position of a draw in one dimension, from the 32-bit word
-// hw alone. The leading 23 bits s = hw >> 9 stand for 1 - u, so the point
-// tracks the FP64 draw of the same word to 2^-23: xn = 50 (1 - (s + 1/2) /
-// 2^23), held exactly in fixed point as p = xn 2^23 = 50 2^23 - 25 - 50 s
-// (a 24x24-bit multiply-add; p is odd, so xn never sits on a bin edge).
-// Returns the bin p >> 23 (always 0..49) and sets one_plus_frac = 1 + frac,
-// the float whose mantissa is the 23 fraction bits of p.
-__device__ __forceinline__ int mixed_position(uint32_t hw, float& one_plus_frac) {
-  const uint32_t s = hw >> 9;
-  const uint32_t p = 419430375u - 50u * s;  // 50 * 2^23 - 25 - 50 s
-  one_plus_frac = __uint_as_float((p & 0x7FFFFFu) | 0x3F800000u);
-  return (int)(p >> 23);
+// hw alone, in the mirrored coordinate m = 50 - xn (xn = 50 (1 - u),
+// vflow.py:117, counted from the top of the unit interval). With u taken as
+// (hw + 1/2) / 2^32, m 2^23 = (50 hw + 25) / 2^9; its integer part
+// q = floor((50 hw + 25) / 2^9) is the high word of the 64-bit
+// hw * 50 2^23 + 25 2^23, one v_mad_u64_u32, so the point tracks the FP64
+// draw of the same word to 2^-23 of a bin. Returns the mirrored bin
+// k = q >> 23 (always 0..49; the bin is 49 - k) and sets one_plus_mfrac to
+// 1 + mfrac, the float whose mantissa is the 23 fraction bits of q: the
+// fraction within bin 49 - k is 1 - mfrac (mfrac = 0: the bin's upper edge).
+__device__ __forceinline__ int mixed_mirrored(uint32_t hw, float& one_plus_mfrac) {
+  const uint32_t q = (uint32_t)(((uint64_t)hw * (50ull << 23) + (25ull << 23)) >> 32);
+  one_plus_mfrac = __uint_as_float((q & 0x7FFFFFu) | 0x3F800000u);
+  return (int)(q >> 23);
 }
 
 // ---------------------------------------------------------------------------

@@ -4,12 +4,13 @@
 // Per (dimension, bin) the LDS table holds the map frac -> y = P + Q frac of
 // the integrand coordinate (Integrand::fold, integration limits folded in)
 // and, in FP64, the Jacobian factor W = dx * 50 (vflow.py:73, 78). The FP32
-// table of the mixed path takes 1 + frac (mixed_position), so it stores
-// (P - Q, Q), and has no W column: W = Q * c_j with the per-dimension
-// constant c_j = 50 / (xdelta_j * fold scale_j) (jacobian_scale), so an
-// event multiplies its slopes Q_j and, after each group of four dimensions,
-// the product of that group's c_j (jacobian_groups), instead of one more LDS
-// read per dimension.
+// table of the mixed path is indexed by the mirrored bin k = 49 - bin and
+// takes 1 + mfrac, mfrac = 1 - frac (mixed_mirrored), so entry k holds
+// (P + 2Q, -Q): y = (P + 2Q) - Q (1 + mfrac) = P + Q frac. It has no W
+// column: W = Q * c_j with the per-dimension constant c_j = 50 / (xdelta_j *
+// fold scale_j) (jacobian_scale), so an event multiplies its slopes Q_j
+// and, after each group of four dimensions, the product of that group's c_j
+// (jacobian_groups), instead of one more LDS read per dimension.
 #pragma once
 
 #include <type_traits>
@@ -36,8 +37,13 @@ __device__ __forceinline__ void build_table(const mv_integrand& fn, const double
       Q = dx * xdelta[j];
     }
     Integrand<T, FAM>::fold(fn, j, P, Q);
-    tab[i].x = sizeof(T) == 4 ? (T)(P - Q) : (T)P;
-    tab[i].y = (T)Q;
+    if constexpr (sizeof(T) == 4) {  // mirrored: entry 49 - b
+      tab[i + kBins - 1 - 2 * b].x = (T)(P + 2.0 * Q);
+      tab[i + kBins - 1 - 2 * b].y = (T)(-Q);
+    } else {
+      tab[i].x = (T)P;
+      tab[i].y = (T)Q;
+    }
     if (tw) tw[i] = (T)(dx * kFBins);  // vflow.py:78
   }
 }
@@ -176,7 +182,7 @@ __device__ __forceinline__ double event_f64(Integrand<double, FAM>& F, uint32_t 
 }
 
 // One event, mixed precision: FP32 map and integrand from the high word of
-// each draw (mixed_position), FP64 result.
+// each draw (mixed_mirrored), FP64 result.
 template <int FAM, bool TRAIN, bool PAIRS = false, int INJ = -1>
 __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,
                                               uint32_t draw, Key key, int groups, int d,
@@ -196,11 +202,11 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
         const int j = 4 * g + q;
         if (j < d) {
           float f1;
-          const int ind = mixed_position(injected ? inj[j] : word(h, q), f1);
-          const float2 t = tab[j * kBins + ind];
+          const int k = mixed_mirrored(injected ? inj[j] : word(h, q), f1);
+          const float2 t = tab[j * kBins + k];
           F.add_folded(j, fmaf(t.y, f1, t.x));
-          w = w * t.y;
-          if (TRAIN) pk[g] |= (uint32_t)ind << (8 * q);
+          w = w * -t.y;
+          if (TRAIN) pk[g] |= (uint32_t)(kBins - 1 - k) << (8 * q);
         }
       }
       w = w * cs.g[g];
@@ -214,16 +220,17 @@ __device__ __forceinline__ double event_mixed(Integrand<float, FAM>& F, uint32_t
 }
 
 // One dimension of a mixed event: position of draw word hw in dimension j,
-// the integrand coordinate into F, the map slope into w; returns the bin.
+// the integrand coordinate into F, the map slope into w; returns the
+// mirrored bin 49 - bin.
 template <int FAM>
 __device__ __forceinline__ uint32_t mixed_dim(Integrand<float, FAM>& F, int j, uint32_t hw,
                                               const float2* tab, float& w) {
   float f1;
-  const int b = mixed_position(hw, f1);
-  const float2 t = tab[j * kBins + b];
+  const int k = mixed_mirrored(hw, f1);
+  const float2 t = tab[j * kBins + k];
   F.add_folded(j, fmaf(t.y, f1, t.x));
-  w = w * t.y;
-  return (uint32_t)b;
+  w = w * -t.y;
+  return (uint32_t)k;
 }
 
 template <int FAM>
@@ -242,7 +249,9 @@ __device__ __forceinline__ void pair_dim(Integrand<float, FAM>& F, int j, uint32
 // cell in each pair table is built while the dimensions are mapped (no bin
 // packing), groups of four dimensions that are all present skip the
 // per-dimension checks, and INJ (injected words instead of Philox) is a
-// template argument rather than a test per dimension. Adds tmp^2 to the pair
+// template argument rather than a test per dimension. The pair tables are
+// mirrored like the map table: bins (b, b') of a pair add into cell
+// (49 - b) * 50 + 49 - b' (finish_pairs with MIRROR). Adds tmp^2 to the pair
 // tables at H and returns tmp.
 template <int FAM, bool INJ>
 __device__ __forceinline__ double event_mixed_pairs(Integrand<float, FAM>& F, uint32_t c0, uint32_t c1,

@@ -3,8 +3,8 @@
 // kernel of pairs.hip does not apply), compiled per (integrand family,
 // train, n_dim) so that the event body is straight-line code.
 //
-// Per event: one Philox word per dimension (mixed_position), FP32 map and
-// integrand from the (P - Q, Q) table in LDS, FP64 sums; when training,
+// Per event: one Philox word per dimension (mixed_mirrored), FP32 map and
+// integrand from the mirrored (P + 2Q, -Q) table in LDS, FP64 sums; when training,
 // (w f)^2 into the wave's private LDS histogram. Eight waves per SIMD
 // (amdgpu_waves_per_eu): more resident waves overlap the histogram atomics
 // with the Philox VALU work.

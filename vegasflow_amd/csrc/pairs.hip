@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kPairBlock) void vegas_fused_pairs(StepK s, mv_inte
       acc2 += tmp * tmp;
     }
   }
-  finish_pairs<kPairBlock>(acc, acc2, H, D, group_row(R));
+  finish_pairs<kPairBlock, true, true>(acc, acc2, H, D, group_row(R));
 }
 
 namespace {

@@ -193,8 +193,9 @@ __device__ __forceinline__ void finish_row(double acc, double acc2, const double
 
 // Workgroup end of the pair-histogram producers (BS threads): sums as in
 // finish_row, histogram bins from the pair tables (pair_marginal_half). SUMS =
-// false leaves columns 0 and 1 alone (VEGAS+ sums per hypercube).
-template <int BS, bool SUMS = true>
+// false leaves columns 0 and 1 alone (VEGAS+ sums per hypercube). MIRROR:
+// the tables hold bin b of each dimension at 49 - b (the mixed kernels).
+template <int BS, bool SUMS = true, bool MIRROR = false>
 __device__ __forceinline__ void finish_pairs(double acc, double acc2, const double* H, int d,
                                              double* row) {
   constexpr int W = BS / 64;
@@ -218,7 +219,8 @@ __device__ __forceinline__ void finish_pairs(double acc, double acc2, const doub
   // two threads per bin (adjacent lanes), each summing half of its cells
   for (int t = threadIdx.x; t < 2 * d * kBins; t += BS) {
     const int m = t >> 1, half = t & 1;
-    const double part = pair_marginal_half(H, m, half);
+    // bin b of dimension j: table index j * 50 + (49 - b) when mirrored
+    const double part = pair_marginal_half(H, MIRROR ? (m / kBins) * (2 * kBins) + kBins - 1 - m : m, half);
     const double h = part + __shfl_xor(part, 1, 64);  // half 0 + half 1 in both lanes
     if (half == 0 && h != 0.0) row_add(&row[2 + m], h);
   }

@@ -122,7 +122,7 @@ __device__ __forceinline__ void stage_store_bytes(const uint8_t* st, uint8_t* __
 
 // vflow.py:87-126 + 33-83, monte_carlo.py:249-275. FP64: the reference's
 // operation order (x = lo + dx * frac with two roundings). Mixed: the
-// positions of mixed_position, map in FP32.
+// positions of mixed_mirrored, map in FP32.
 template <bool MIXED, bool STAGED>
 __global__ __launch_bounds__(kBlock) void vegas_sample(StepK s, const double* __restrict__ grid,
                                                        void* __restrict__ xo_, void* __restrict__ wo_,
@@ -158,10 +158,10 @@ __global__ __launch_bounds__(kBlock) void vegas_sample(StepK s, const double* __
             if constexpr (MIXED) {
               const uint32_t* inj = static_cast<const uint32_t*>(s.inject);
               float f1;
-              ind = mixed_position(inj ? inj[i * d + j] : word(h, q), f1);
+              ind = kBins - 1 - mixed_mirrored(inj ? inj[i * d + j] : word(h, q), f1);
               const double lo = grid[j * (kBins + 1) + ind];
               const double dx = grid[j * (kBins + 1) + ind + 1] - lo;
-              x = fmaf((float)dx, f1 - 1.0f, (float)lo);  // f1 - 1 is exact
+              x = fmaf((float)dx, 2.0f - f1, (float)lo);  // frac = 2 - f1 = 1 - mfrac is exact
               w = w * (float)(dx * kFBins);
               if (limits) x = fmaf(x, (float)s.xdelta[j], (float)s.xmin[j]);
             } else {
