@@ -42,8 +42,8 @@ TRAFFIC_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_pmc.md"
 # `bench.py --only issue` measures on gfx950 (2 cycles per wave64 FP32 add,
 # multiply or FMA, 8 per v_exp_f32, 4 for every other kind), set against
 # 4 SIMDs per CU at the 2.4 GHz engine clock (MI355X datasheet)
-VALU_INSTS = {"mixed": 2.473e7, "f64": 5.107e7}
-VALU_BUSY_CYCLES = {"mixed": 581.2, "f64": 1307.3}
+VALU_INSTS = {"mixed": 2.351e7, "f64": 5.107e7}
+VALU_BUSY_CYCLES = {"mixed": 549.9, "f64": 1307.3}
 VALU_SOURCE = "profiles/vegas_fused_gauss_d8_1e7_valu_mix.md"
 CLOCK_HZ = 2.4e9
 # host cores allotted to one GPU of the node (CPU worker pools are sized to it)
