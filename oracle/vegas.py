@@ -70,15 +70,21 @@ def hi_words(seed, draw_id, n, n_dim, event_offset=0):
     return words[:, :n_dim].copy()
 
 
+MIXED_MUL = 50 * 2 ** 23 - 10
+MIXED_ADD = 5 * 2 ** 32 + MIXED_MUL // 2
+
+
 def mixed_position(words):
     """Bin and fraction of the mixed-precision path (the engine's own form;
-    the reference has only FP64/FP32 uniforms). With u taken as
-    (word + 1/2) / 2^32, the mirrored coordinate m = 50 - xn = 50 u is held
-    to 23 fraction bits as q = floor((50 word + 25) / 2^9); the point is
+    the reference has only FP64/FP32 uniforms). The mirrored coordinate
+    m = 50 - xn = 50 u is held to 23 fraction bits as
+    q = floor((K (word + 1/2) + 5 2^32) / 2^32), K = 50 2^23 - 10, which runs
+    from 5 to 50 2^23 - 6: u = q / (50 2^23) stays inside
+    [TECH_CUT, 1 - TECH_CUT] (monte_carlo.py:264-266) and the point is
     xn = 50 - q / 2^23 exactly. Returns (bin = 49 - (q >> 23),
     frac = 1 - (q mod 2^23) / 2^23), i.e. xn = bin + frac exactly, with
     frac in (0, 1] (frac = 1: the bin's upper edge)."""
-    q = (np.asarray(words).astype(np.int64) * 50 + 25) >> 9
+    q = (np.asarray(words).astype(np.int64) * MIXED_MUL + MIXED_ADD) >> 32
     mfrac = (q & (2 ** 23 - 1)).astype(np.float64) * 2.0 ** -23
     return (49 - (q >> 23)).astype(np.int32), 1.0 - mfrac
 

@@ -277,28 +277,33 @@ def test_torch_cpu_baseline_matches_the_oracle_on_the_same_draws():
 
 
 def test_mixed_position_exact_and_tracks_the_f64_draws():
-    """The mixed path's (bin, frac) is xn = 50 - floor((50 w + 25) / 2^9) /
-    2^23 exactly (rational brute force), inside (0, 50] with frac in (0, 1],
-    within 2^-23 of 50 (1 - (w + 1/2) / 2^32), and lands in the bin of the
-    FP64 draw of the same word but for points within ~2^-23 of an edge."""
+    """The mixed path's (bin, frac) is xn = 50 - q / 2^23 with
+    q = floor(((50 2^23 - 10) (w + 1/2) + 5 2^32) / 2^32) exactly (rational
+    brute force); u = q / (50 2^23) lies inside [TECH_CUT, 1 - TECH_CUT] like
+    the FP64 draws; xn is within 2^-21 of the FP64 position of the same word
+    (50 (1 - u) with u = TECH_CUT + (w + 1/2) / 2^32 (1 - 2 TECH_CUT)); and
+    the bin is the FP64 draw's but for points within ~2^-21 of an edge."""
     from fractions import Fraction
 
     words = ora.hi_words(9, 2, 4000, 3).reshape(-1)
     extreme = np.array([0, 1, 9, 10, 511, 512, 0x7FFFFFFF, 0xFFFFFDFF, 0xFFFFFFFF], dtype=np.uint32)
     words = np.concatenate([words, extreme])
     ind, frac = ora.mixed_position(words)
+    K = 50 * 2 ** 23 - 10
+    tc = Fraction(ora.TECH_CUT)
     for k in list(range(0, words.size, 37)) + list(range(words.size - extreme.size, words.size)):
         w = int(words[k])
-        q = (50 * w + 25) // 512
+        q = (K * (2 * w + 1) + 10 * 2 ** 32) // 2 ** 33
         xn = 50 - Fraction(q, 2 ** 23)
         assert ind[k] == 49 - q // 2 ** 23 and Fraction(float(frac[k])) == xn - ind[k]
-        assert abs(xn - 50 * (1 - Fraction(2 * w + 1, 2 ** 33))) < Fraction(1, 2 ** 23)
+        u = Fraction(q, 50 * 2 ** 23)
+        assert tc <= u <= 1 - tc
+        u64 = tc + Fraction(2 * w + 1, 2 ** 33) * (1 - 2 * tc)
+        assert abs(xn - 50 * (1 - u64)) < Fraction(1, 2 ** 21)
     assert ind.min() >= 0 and ind.max() <= 49
-    assert frac.min() > 0 and frac.max() <= 1
-    # word 0 sits on the top edge of bin 49 (x = 1); the largest word just
-    # above 0
+    assert frac.min() > 0 and frac.max() < 1
     i0, f0 = ora.mixed_position(np.array([0, 0xFFFFFFFF], dtype=np.uint32))
-    assert (i0[0], f0[0]) == (49, 1.0) and i0[1] == 0 and 0 < f0[1] <= 2.0 ** -22
+    assert (i0[0], f0[0]) == (49, 1.0 - 5 * 2.0 ** -23) and (i0[1], f0[1]) == (0, 6 * 2.0 ** -23)
     n, d = 20000, 4
     _, _, ind64 = ora.vegas_map(ora.uniform_f64(5, 1, n, d), ora.initial_grid(d))
     ind32, _ = ora.mixed_position(ora.hi_words(5, 1, n, d))

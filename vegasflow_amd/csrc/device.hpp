@@ -68,16 +68,20 @@ __device__ __forceinline__ double draw_f64(uint32_t hi, uint32_t lo) {
 
 // Mixed-precision position of a draw in one dimension, from the 32-bit word
 // hw alone, in the mirrored coordinate m = 50 - xn (xn = 50 (1 - u),
-// vflow.py:117, counted from the top of the unit interval). With u taken as
-// (hw + 1/2) / 2^32, m 2^23 = (50 hw + 25) / 2^9; its integer part
-// q = floor((50 hw + 25) / 2^9) is the high word of the 64-bit
-// hw * 50 2^23 + 25 2^23, one v_mad_u64_u32, so the point tracks the FP64
-// draw of the same word to 2^-23 of a bin. Returns the mirrored bin
-// k = q >> 23 (always 0..49; the bin is 49 - k) and sets one_plus_mfrac to
-// 1 + mfrac, the float whose mantissa is the 23 fraction bits of q: the
-// fraction within bin 49 - k is 1 - mfrac (mfrac = 0: the bin's upper edge).
+// vflow.py:117, counted from the top of the unit interval), to 23 fraction
+// bits: q = floor((K (hw + 1/2) + 5 2^32) / 2^32) with K = 50 2^23 - 10, the
+// high word of one 64-bit multiply-add (v_mad_u64_u32). q runs from 5 to
+// 50 2^23 - 6, so u = q / (50 2^23) stays inside [TECH_CUT, 1 - TECH_CUT]
+// like the reference's draws (monte_carlo.py:264-266), and the point tracks
+// the FP64 draw of the same word to about 2^-22 of a bin. Returns the
+// mirrored bin k = q >> 23 (always 0..49; the bin is 49 - k) and sets
+// one_plus_mfrac to 1 + mfrac, the float whose mantissa is the 23 fraction
+// bits of q: the fraction within bin 49 - k is 1 - mfrac (mfrac = 0: the
+// bin's upper edge).
 __device__ __forceinline__ int mixed_mirrored(uint32_t hw, float& one_plus_mfrac) {
-  const uint32_t q = (uint32_t)(((uint64_t)hw * (50ull << 23) + (25ull << 23)) >> 32);
+  constexpr uint64_t kMul = (50ull << 23) - 10ull;
+  constexpr uint64_t kAdd = (5ull << 32) + (kMul >> 1);
+  const uint32_t q = (uint32_t)(((uint64_t)hw * kMul + kAdd) >> 32);
   one_plus_mfrac = __uint_as_float((q & 0x7FFFFFu) | 0x3F800000u);
   return (int)(q >> 23);
 }
